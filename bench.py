@@ -1,0 +1,148 @@
+"""Throughput of the PBD cloth substep with Morton/LBVH self-collision on one MI355X.
+
+Workload: one 256x256-vertex cloth (65,536 vertices, 130,050 triangles, 260,610
+structural+shear distance constraints, 194,565 dihedral bends) dropped from 0.5 m
+onto a static sphere of radius 0.3 m, thickness 0.002 m, dt 1/120 s with 2
+substeps per frame (substep dt 1/240 s), 15 Jacobi iterations, 30-bit Morton codes
+in the centroid box, LBVH self-collision with VF and EE contacts every substep.
+A step is one substep; the default run times the full 600 frames (1,200 substeps).
+
+Prints one JSON line. ``roofline`` covers the dominant kernel (the fused Jacobi
+iteration, 15 launches per substep) with its algorithmic bytes per launch
+(36 N + 16 S + 24 Bd + 48 C) over its average launch time measured with HIP events
+on the body's stream; ``cpu_baseline`` times the FP32 CPU oracle on a bounded
+sample of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PBD substeps/s at 256^2 self-colliding cloth"
+
+
+def algorithmic_bytes(n, s, bd, t, iters, contacts, pairs):
+    per_iter = 36 * n + 16 * s + 24 * bd + 48 * contacts
+    substep = 112 * n + iters * per_iter + 300 * t + 32 * n + 8 * pairs
+    return per_iter, substep
+
+
+def cpu_baseline(scene, budget_s: float):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the CPU restatement, used only as the reported baseline
+
+    o = oracle.OracleSim(scene, 32)
+    done, elapsed = 0, 0.0
+    while elapsed < budget_s and done < 64:
+        elapsed += o.time_steps(1)
+        done += 1
+    return dict(value=done / elapsed, unit="substeps/s", cores=1, kind="port",
+                sample=f"first {done} substeps of the same 256x256 sphere-drop workload, "
+                       f"FP32 oracle (C++ restatement of the reference path), 1 thread, "
+                       f"{elapsed:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=150)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the batched and single-cloth paths shard by cloth "
+              f"with no collectives, so --gpus {args.gpus} is not measured here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from licenta_amd import configs
+    from licenta_amd._abi import lib
+    from licenta_amd.body import ClothBody
+    import ctypes as C
+
+    scene = configs.config_scene(1, args.size)
+    body = ClothBody.from_scene(scene)
+    body.fixed_update(args.warmup)
+    body.sync()
+    t0 = time.perf_counter()
+    body.fixed_update(args.steps)
+    body.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    counts = body.counts()
+    warn = body.warnings()
+    # dominant kernel, timed live with HIP events on the body's stream
+    avg = C.c_float()
+    rc = lib().cm_time_kernel(body.h, 0, args.kernel_reps, C.byref(avg))
+    if rc:
+        raise RuntimeError("cm_time_kernel failed")
+    iters = scene["params"]["solver_iterations"]
+    contacts = counts["vf"] + counts["ee"]
+    per_iter, per_substep = algorithmic_bytes(counts["n"], counts["s"], counts["b"], counts["t"],
+                                              iters, contacts, counts["pairs"])
+    solve_ms = float(avg.value)
+    achieved = per_iter / (solve_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "k_solve_sphere_drop_256_traffic.json")
+    if os.path.exists(tf) and args.size == 256:
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    out = dict(
+        metric=METRIC,
+        value=1e3 / ms_per_step,
+        unit="substeps/s",
+        n_gpus=1,
+        steps=args.steps,
+        warmup=args.warmup,
+        ms_per_step=ms_per_step,
+        higher_is_better=True,
+        scaling="weak",
+        vs_baseline=None,
+        dtype="f32",
+        data="synthetic",
+        config=dict(workload=f"sphere_drop_{args.size}x{args.size}", vertices=counts["n"],
+                    triangles=counts["t"], distance_constraints=counts["s"],
+                    bend_constraints=counts["b"], iterations=iters, substep_dt=1 / 240,
+                    morton_bits=30, parallelism="single cloth, one GPU"),
+        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=achieved / HBM_PEAK_GBS, traffic=traffic,
+                      kernel="k_solve (fused Jacobi iteration)", bytes_per_launch=per_iter,
+                      avg_launch_ms=solve_ms),
+        vertex_updates_per_s=counts["n"] * 1e3 / ms_per_step,
+        substep_roofline_frac=per_substep / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        last_substep=dict(pairs=counts["pairs"], vf=counts["vf"], ee=counts["ee"],
+                          warnings=warn),
+    )
+    body.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(configs.config_scene(1, args.size), args.cpu_budget)
+        out["cpu_baseline"]["cpu_model"] = _cpu_model()
+        out["cpu_baseline"]["nproc"] = os.cpu_count()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,226 @@
+/*
+ * clothmi.h — C ABI of the MI355X-native PBD cloth substep with Morton/LBVH
+ * self-collision.
+ *
+ * The entry points replace the reference's simulation-object interface and its
+ * collision-method free functions (all paths relative to
+ * LicentaV2/LicentaV2/ in Tureyz/Licenta):
+ *
+ *   cm_create          <- Physics::CudaDeformableBody::CudaDeformableBody
+ *                         (Physics/CudaDeformableBody.h:23-24,
+ *                          Physics/CudaDeformableBody.cpp:30-285)
+ *   cm_step            <- Physics::CudaDeformableBody::FixedUpdate, one call per
+ *                         substep (Physics/CudaDeformableBody.cpp:550-627)
+ *   cm_read_positions  <- Physics::CudaDeformableBody::Update / FinalVertUpdate
+ *                         (Physics/CudaDeformableBody.cpp:629-638, 473-495)
+ *   cm_read_normals    <- DeformableUtils::UpdateVertexNormals + FinalVertUpdate
+ *                         (Physics/CudaDeformableUtils.cu:190-249,317-344)
+ *   cm_set_sphere      <- Physics::CudaDeformableBody::SetSpherePos
+ *                         (Physics/CudaDeformableBody.cpp:640-643)
+ *   cm_destroy         <- Physics::CudaDeformableBody::~CudaDeformableBody
+ *                         (Physics/CudaDeformableBody.cpp:317-320)
+ *   cm_query_pairs     <- CudaBVH::GetAABBCollisions + CubWrap::SelectFlagged
+ *                         (Collision/CudaBVH.cu:9-113, Collision/CudaCollisionDetection.cu:17)
+ *   cm_query_contacts  <- DeformableUtils::CCDTriangleTests output vfContacts/eeContacts
+ *                         (Collision/CudaCollisionDetection.cu:223-283)
+ *   cm_tri_update      <- DeformableUtils::_UpdateTrianglesContinuous
+ *                         (Physics/CudaDeformableUtils.cu:120-151)
+ *   cm_lbvh_sort       <- DeformableUtils::SortMortons -> CubWrap::SortByKey
+ *                         (Physics/CudaDeformableUtils.cu:346-349, Core/CubWrappers.cuh:141-156)
+ *   cm_lbvh_build      <- CudaBVH::GenerateBVH2 (Collision/CudaBVH.cu:152-242)
+ *   cm_lbvh_refit      <- CudaBVH::ComputeTreeAABBs (Collision/CudaBVH.cu:115-149)
+ *   cm_lbvh_pairs      <- CudaBVH::GetAABBCollisions (Collision/CudaBVH.cu:9-113)
+ *
+ * Conventions: every function returns 0 (CM_OK) or a negative CM_ERR_* code and
+ * never exits or aborts; cm_last_error() gives the message of the last failure
+ * on the calling thread. Host arrays are copied during the call and never
+ * retained. Device pointers (d_*) belong to the caller and must be valid for
+ * the given stream; the unit-level cm_tri_update/cm_lbvh_* calls are
+ * stream-ordered and return without synchronising. A cm_body owns its device
+ * memory and one HIP stream; one body must not be used from two threads at
+ * once, separate bodies may.
+ */
+#ifndef CLOTHMI_H
+#define CLOTHMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define CM_OK 0
+#define CM_ERR_ARG (-1)        /* invalid argument / size                    */
+#define CM_ERR_HIP (-2)        /* HIP runtime error                          */
+#define CM_ERR_ALLOC (-3)      /* device allocation failed                   */
+#define CM_ERR_STATE (-4)      /* call not valid in the current state        */
+#define CM_ERR_NO_DEVICE (-5)  /* no usable GPU                              */
+
+/* warning bits accumulated on a body (cm_get_warnings) */
+#define CM_WARN_STACK_OVERFLOW (1u << 0)  /* traversal stack (64) full: right subtree skipped */
+#define CM_WARN_TRAVERSAL_LIMIT (1u << 1) /* traversal visit bound reached for some leaf     */
+#define CM_WARN_BROKEN_TREE (1u << 2)     /* midpoint split on tied codes made a node shared */
+#define CM_WARN_PAIR_CAP (1u << 3)        /* some leaf dropped pairs past the per-leaf cap    */
+#define CM_WARN_CANDIDATE_CAP (1u << 4)   /* candidate/contact buffer full: list truncated    */
+#define CM_WARN_REFIT_LIMIT (1u << 5)     /* refit climb bound reached                         */
+
+/* ---- feature flags (cm_params.flags) ------------------------------------ */
+#define CM_F_SELF_COLLISION (1u << 0) /* Morton/LBVH/CCD contacts every substep */
+#define CM_F_GROUND (1u << 1)         /* ground plane at ground_height          */
+#define CM_F_WORLD_CLAMP (1u << 2)    /* clamp predicted positions to the box   */
+#define CM_F_DAMPING (1u << 3)        /* global + rigid-mode velocity damping   */
+#define CM_F_LRA (1u << 4)            /* long-range attachments to pinned verts */
+#define CM_F_SPHERE (1u << 5)         /* static/moving analytic sphere          */
+#define CM_F_EE_CONTACTS (1u << 6)    /* edge-edge contacts in the solver       */
+#define CM_F_FRICTION (1u << 7)       /* velocity friction after integration    */
+/* Evaluate the dihedral bend (Physics/CudaPBD.cu:1024-1147) in FP64 from the FP32
+ * positions and round the corrections to FP32. On absolute positions the
+ * reference formula cancels catastrophically in FP32 wherever two vertices are
+ * nearly collinear with the origin; FP64 gives the formula's exact value there. */
+#define CM_F_BEND_FP64 (1u << 8)
+
+/* ---- reference-compatibility bits (cm_params.ref_compat) ----------------- */
+/* Leaf AABBs stay in pre-sort triangle slots after the Morton sort
+ * (Physics/CudaDeformableUtils.cu:146-147 with Core/CubWrappers.cuh:141-156). */
+#define CM_RC_STALE_LEAF_AABB (1u << 0)
+/* Dihedral bend on absolute positions with rest angle pi
+ * (Physics/CudaPBD.cu:1040-1046, 366-368); off = p1-relative Mueller bend with
+ * the initial dihedral angle as rest angle. */
+#define CM_RC_BEND_ABS (1u << 1)
+/* FindSplit returns (first+last)>>1 when the end codes are equal
+ * (Collision/CudaBVH.cu:219-220); off = index-augmented Karras split. */
+#define CM_RC_MIDPOINT_SPLIT (1u << 2)
+#define CM_RC_ALL (CM_RC_STALE_LEAF_AABB | CM_RC_BEND_ABS | CM_RC_MIDPOINT_SPLIT)
+
+/* POD replacement for Physics::ClothParams (Physics/Structs.h:230-268). */
+typedef struct cm_params {
+  float timestep;          /* substep dt in seconds                              */
+  float gravity[3];        /* m/s^2                                              */
+  float k_stretch;         /* distance stiffness when k_dist is NULL             */
+  float k_bend;            /* bend stiffness when k_bend is NULL                 */
+  float thickness;         /* collision thickness (no override, unlike :86)     */
+  int32_t solver_iterations;
+  float ground_height;     /* ground plane y (reference: worldMin.y)             */
+  float world_min[3];      /* clamp box (reference: WORLD_MIN/WORLD_MAX)         */
+  float world_max[3];
+  float global_vel_damp;   /* kg: v *= (1-kg)                                    */
+  float k_damp;            /* rigid-mode damping coefficient                     */
+  float strain_limit;      /* LRA rest-length slack                              */
+  float k_friction;
+  float pinned_mass;       /* mass used for pinned vertices in damping (ref 1000)*/
+  float sphere_center[3];
+  float sphere_radius;
+  int32_t morton_bits;     /* 63 (reference) or 30                               */
+  int32_t morton_normalize;/* 0: raw coords clamped to [0,1]; 1: centroid box    */
+  int32_t pair_cap;        /* per-leaf triangle-pair cap (reference 64)          */
+  uint32_t flags;          /* CM_F_*                                             */
+  uint32_t ref_compat;     /* CM_RC_*                                            */
+  int32_t reserved[7];
+} cm_params;
+
+/* Same fields as Physics::PrimitiveContact (Physics/Structs.h:313-336), 48 B. */
+typedef struct cm_contact {
+  int32_t v[4];
+  float w[4];
+  float t;
+  float n[3];
+} cm_contact;
+
+typedef struct cm_body cm_body;
+
+/* Internal buffers readable with cm_get_buffer (element type in brackets). */
+enum cm_buffer {
+  CM_BUF_X = 0,          /* [float4 N] carried positions (m_dPrevPositions)      */
+  CM_BUF_P = 1,          /* [float4 N] predicted positions (m_dPositions)        */
+  CM_BUF_V = 2,          /* [float4 N] velocities                                */
+  CM_BUF_TRI = 3,        /* [int4 T] triangles in current (sorted) order, .w = feature bits */
+  CM_BUF_KEYS = 4,       /* [uint64 T] sorted Morton keys                        */
+  CM_BUF_PERM = 5,       /* [uint32 T] sort permutation (pre-sort slot of sorted i) */
+  CM_BUF_LEFTS = 6,      /* [int32 T-1]                                          */
+  CM_BUF_RIGHTS = 7,     /* [int32 T-1]                                          */
+  CM_BUF_PARENTS = 8,    /* [int32 2T-1]                                         */
+  CM_BUF_RMLL = 9,       /* [int32 T-1]                                          */
+  CM_BUF_RMLR = 10,      /* [int32 T-1]                                          */
+  CM_BUF_AABB_MIN = 11,  /* [float4 2T-1] internal nodes then leaves             */
+  CM_BUF_AABB_MAX = 12,  /* [float4 2T-1]                                        */
+  CM_BUF_PAIRS = 13,     /* [int2 P] compacted triangle pairs                    */
+  CM_BUF_VF = 14,        /* [cm_contact C_vf]                                    */
+  CM_BUF_EE = 15,        /* [cm_contact C_ee]                                    */
+  CM_BUF_UNUSED_KEYS_PRESORT = 16
+};
+
+void cm_default_params(cm_params* params);
+const char* cm_last_error(void);
+
+/* Body lifecycle. xyz: 3N floats; tri: 3T vertex ids; pinned: N bytes or NULL
+ * (NULL: pinned where inv_mass == 0); inv_mass: N floats; dist_pairs: 2*n_dist
+ * vertex ids or NULL (NULL: one distance constraint per unique mesh edge, as
+ * Physics/CudaPBD.cu:111-142); k_dist: per distance constraint or NULL;
+ * k_bend: per bend constraint or NULL. */
+int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32_t* tri,
+              int32_t t, const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
+              int32_t n_dist, const float* k_dist, const float* k_bend, cm_body** out);
+int cm_destroy(cm_body* body);
+
+/* Runs `substeps` substeps on the body's stream (graph-replayed); does not sync. */
+int cm_step(cm_body* body, int32_t substeps);
+int cm_sync(cm_body* body);
+
+int cm_read_positions(cm_body* body, float* xyz);   /* 3N, syncs */
+int cm_read_velocities(cm_body* body, float* xyz);  /* 3N, syncs */
+int cm_read_normals(cm_body* body, float* xyz);     /* 3N, syncs */
+int cm_set_sphere(cm_body* body, const float center[3], float radius);
+/* Replace the state. x, p, v: 3N floats each (p may be NULL: p = x);
+ * tri: 4T ints (current triangle order with feature bits) or NULL. */
+int cm_set_state(cm_body* body, const float* x, const float* p, const float* v,
+                 const int32_t* tri);
+int cm_get_counts(cm_body* body, int64_t* counts /* [n, t, s, bd, pairs, vf, ee] */);
+int cm_get_buffer(cm_body* body, int32_t which, void* host, int64_t capacity_bytes,
+                  int64_t* count);
+int cm_get_warnings(cm_body* body, uint32_t* warnings);
+int cm_query_pairs(cm_body* body, int32_t* pairs, int32_t capacity, int32_t* n_pairs);
+int cm_query_contacts(cm_body* body, cm_contact* vf, int32_t cap_vf, int32_t* n_vf,
+                      cm_contact* ee, int32_t cap_ee, int32_t* n_ee);
+
+/* Kernel timing for roofline reporting, in the spirit of the reference's stage
+ * timers (Simulation/GPUBenchmark.cu:14-77): launches kernel `which`
+ * (CM_KERNEL_*) `reps` times on the body's stream between two HIP events and
+ * returns the average duration in milliseconds. CM_KERNEL_SOLVE re-runs Jacobi
+ * iterations on the current predicted positions (it changes them). */
+#define CM_KERNEL_SOLVE 0
+#define CM_KERNEL_SUBSTEP 1
+int cm_time_kernel(cm_body* body, int32_t which, int32_t reps, float* avg_ms);
+
+/* ---- unit-level LBVH entry points on caller device buffers -------------- */
+/* Swept leaf AABBs (written to leaf slots [T-1, 2T-1) of d_aabb_min/max) and
+ * Morton keys for triangles d_tri in their current order. */
+int cm_tri_update(int32_t n, int32_t t, const void* d_x /*float4 N*/, const void* d_p /*float4 N*/,
+                  const void* d_tri /*int4 T*/, float thickness, int32_t morton_bits,
+                  int32_t morton_normalize, void* d_aabb_min /*float4 2T-1*/,
+                  void* d_aabb_max, uint64_t* d_keys, void* stream);
+size_t cm_lbvh_sort_temp_bytes(int32_t t);
+/* Stable LSD radix sort of (d_keys, d_vals) by the low `key_bits` bits, in place. */
+int cm_lbvh_sort(int32_t t, int32_t key_bits, uint64_t* d_keys, uint32_t* d_vals, void* d_temp,
+                 size_t temp_bytes, void* stream);
+size_t cm_lbvh_build_temp_bytes(int32_t t);
+int cm_lbvh_build(int32_t t, const uint64_t* d_keys, int32_t midpoint_split, int32_t* d_lefts,
+                  int32_t* d_rights, int32_t* d_parents, uint32_t* d_warn, void* d_temp,
+                  size_t temp_bytes, void* stream);
+int cm_lbvh_refit(int32_t t, const int32_t* d_lefts, const int32_t* d_rights,
+                  const int32_t* d_parents, void* d_aabb_min, void* d_aabb_max, int32_t* d_rmll,
+                  int32_t* d_rmlr, uint32_t* d_warn, void* d_temp, size_t temp_bytes,
+                  void* stream);
+size_t cm_lbvh_pairs_temp_bytes(int32_t t, int32_t cap);
+/* Pairs (i, j), j > i, per leaf in depth-first order, at most `cap` per leaf,
+ * compacted in leaf order into d_pairs (capacity T*cap int2); *d_n_pairs set. */
+int cm_lbvh_pairs(int32_t t, const int32_t* d_lefts, const int32_t* d_rights,
+                  const void* d_aabb_min, const void* d_aabb_max, const int32_t* d_rmll,
+                  const int32_t* d_rmlr, int32_t cap, int32_t* d_pairs, int32_t* d_n_pairs,
+                  uint32_t* d_warn, void* d_temp, size_t temp_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLOTHMI_H */

@@ -1,0 +1,143 @@
+"""Host-side mirror of the reference's simulation object.
+
+``ClothBody`` keeps the shape of Physics::CudaDeformableBody
+(LicentaV2/LicentaV2/Physics/CudaDeformableBody.h:18-28):
+
+    constructor(verts, indices, params, fixed_verts)  -> cm_create
+    fixed_update()        one substep                -> cm_step      (FixedUpdate, :550-627)
+    update()              read positions + normals   -> cm_read_*    (Update, :629-638)
+    set_sphere_pos(p, r)                              -> cm_set_sphere (:640-643)
+
+and adds the contact-pair query of the collision pipeline (cm_query_pairs /
+cm_query_contacts) plus access to every internal buffer for parity tests.
+All compute runs in libclothmi.so on the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import CONTACT_DTYPE, check, lib, make_params, ptr
+
+BUF = dict(x=0, p=1, v=2, tri=3, keys=4, perm=5, lefts=6, rights=7, parents=8, rmll=9, rmlr=10,
+           aabb_min=11, aabb_max=12, pairs=13, vf=14, ee=15)
+_BUF_DTYPE = dict(x=(np.float32, 4), p=(np.float32, 4), v=(np.float32, 4), tri=(np.int32, 4),
+                  keys=(np.uint64, 1), perm=(np.uint32, 1), lefts=(np.int32, 1),
+                  rights=(np.int32, 1), parents=(np.int32, 1), rmll=(np.int32, 1),
+                  rmlr=(np.int32, 1), aabb_min=(np.float32, 4), aabb_max=(np.float32, 4),
+                  pairs=(np.int32, 2))
+
+WARN_NAMES = {1: "stack_overflow", 2: "traversal_limit", 4: "broken_tree", 8: "pair_cap",
+              16: "candidate_cap", 32: "refit_limit"}
+
+
+class ClothBody:
+    def __init__(self, xyz, tri, params: dict, fixed_verts=None, inv_mass=None, dist_pairs=None,
+                 k_dist=None, k_bend=None):
+        L = lib()
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3)
+        self.tri = np.ascontiguousarray(tri, dtype=np.uint32).reshape(-1, 3)
+        self.n = len(self.xyz)
+        self.t = len(self.tri)
+        if inv_mass is None:
+            inv_mass = np.ones(self.n, dtype=np.float32)
+        self.inv_mass = np.ascontiguousarray(inv_mass, dtype=np.float32)
+        self.pinned = None if fixed_verts is None else np.ascontiguousarray(fixed_verts, dtype=np.uint8)
+        self.dist_pairs = None if dist_pairs is None else np.ascontiguousarray(dist_pairs, dtype=np.int32)
+        self.k_dist = None if k_dist is None else np.ascontiguousarray(k_dist, dtype=np.float32)
+        self.k_bend = None if k_bend is None else np.ascontiguousarray(k_bend, dtype=np.float32)
+        self.params = dict(params)
+        self._p = make_params(self.params)
+        h = C.c_void_p()
+        nd = 0 if self.dist_pairs is None else len(self.dist_pairs)
+        check(L.cm_create(C.byref(self._p), ptr(self.xyz), self.n, ptr(self.tri), self.t,
+                          ptr(self.pinned), ptr(self.inv_mass), ptr(self.dist_pairs), nd,
+                          ptr(self.k_dist), ptr(self.k_bend), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_scene(cls, scene: dict) -> "ClothBody":
+        return cls(scene["xyz"], scene["tri"], scene["params"], scene.get("pinned"),
+                   scene["inv_mass"], scene.get("dist_pairs"), scene.get("k_dist"),
+                   scene.get("k_bend"))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference-shaped interface ------------------------------------------
+    def fixed_update(self, substeps: int = 1) -> None:
+        check(lib().cm_step(self.h, substeps))
+
+    def update(self):
+        """Readback of positions and vertex normals (Update / FinalVertUpdate)."""
+        return self.positions(), self.normals()
+
+    def set_sphere_pos(self, pos, radius: float) -> None:
+        c = np.ascontiguousarray(pos, dtype=np.float32)
+        check(lib().cm_set_sphere(self.h, ptr(c), float(radius)))
+
+    # ---- state -------------------------------------------------------------------
+    def sync(self):
+        check(lib().cm_sync(self.h))
+
+    def positions(self) -> np.ndarray:
+        out = np.zeros((self.n, 3), dtype=np.float32)
+        check(lib().cm_read_positions(self.h, ptr(out)))
+        return out
+
+    def velocities(self) -> np.ndarray:
+        out = np.zeros((self.n, 3), dtype=np.float32)
+        check(lib().cm_read_velocities(self.h, ptr(out)))
+        return out
+
+    def normals(self) -> np.ndarray:
+        out = np.zeros((self.n, 3), dtype=np.float32)
+        check(lib().cm_read_normals(self.h, ptr(out)))
+        return out
+
+    def set_state(self, x, v, p=None, tri=None) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float32).reshape(-1, 3)
+        tri = None if tri is None else np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 4)
+        check(lib().cm_set_state(self.h, ptr(x), ptr(p), ptr(v), ptr(tri)))
+
+    def counts(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        check(lib().cm_get_counts(self.h, ptr(out)))
+        return dict(zip(("n", "t", "s", "b", "pairs", "vf", "ee"), out[:7].tolist()))
+
+    def warnings(self) -> int:
+        w = C.c_uint32()
+        check(lib().cm_get_warnings(self.h, C.byref(w)))
+        return int(w.value)
+
+    def buffer(self, name: str) -> np.ndarray:
+        L = lib()
+        cnt = C.c_int64()
+        if name in ("vf", "ee"):
+            check(L.cm_get_buffer(self.h, BUF[name], None, 0, C.byref(cnt)))
+            out = np.zeros(cnt.value, dtype=CONTACT_DTYPE)
+            check(L.cm_get_buffer(self.h, BUF[name], ptr(out), out.nbytes, C.byref(cnt)))
+            return out
+        dt, width = _BUF_DTYPE[name]
+        check(L.cm_get_buffer(self.h, BUF[name], None, 0, C.byref(cnt)))
+        out = np.zeros((cnt.value, width) if width > 1 else cnt.value, dtype=dt)
+        check(L.cm_get_buffer(self.h, BUF[name], ptr(out), out.nbytes, C.byref(cnt)))
+        return out
+
+    def pairs(self) -> np.ndarray:
+        return self.buffer("pairs")
+
+    def contacts(self):
+        return self.buffer("vf"), self.buffer("ee")

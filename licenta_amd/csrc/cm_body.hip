@@ -1,0 +1,868 @@
+// cm_body: the simulation object behind the C ABI.
+//
+// Replaces Physics::CudaDeformableBody (LicentaV2/LicentaV2/Physics/CudaDeformableBody.cpp):
+//   cm_create  <- constructor :30-285 with CudaPBD::Init (Physics/CudaPBD.cu:39-370) and
+//                 FeatureList::Init/AssignFeatures (Collision/FeatureList.cu:12-132)
+//   cm_step    <- FixedUpdate :550-627 (one call per substep, recorded once into a
+//                 hipGraph and replayed; no host synchronisation inside a substep)
+//   cm_read_*  <- Update / FinalVertUpdate :629-638, 473-495
+//
+// The constraint topology, slot layout and feature ownership are built on the
+// host once; everything per substep runs on the body's stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "cm_internal.hpp"
+
+namespace cm {
+
+static_assert(sizeof(cm_params) == 144, "cm_params layout is part of the ABI");
+static_assert(sizeof(cm_contact) == 48, "cm_contact layout is part of the ABI");
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+}  // namespace cm
+
+using namespace cm;
+
+struct cm_body {
+  cm_params prm;
+  int n = 0, t = 0, S = 0, B = 0, F = 0, nslots = 0, cap = 0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // state
+  float4 *x = nullptr, *p = nullptr, *p2 = nullptr, *v = nullptr;
+  uint8_t* pinned = nullptr;
+  float* mass = nullptr;
+  // topology
+  int2* sid = nullptr;
+  float2* s_l0kp = nullptr;
+  int4* bid = nullptr;
+  float2* b_kpphi0 = nullptr;
+  int* vprefix = nullptr;
+  uint32_t* vslot = nullptr;
+  int* lra_fixed = nullptr;
+  float* lra_l0 = nullptr;
+  uint8_t* lra_flag = nullptr;
+  float4* ground = nullptr;
+  float *damp_partials = nullptr, *damp_state = nullptr;
+  // collision
+  int4 *tri = nullptr, *tri_alt = nullptr;
+  uint64_t* keys = nullptr;
+  uint32_t* perm = nullptr;
+  void* sort_temp = nullptr;
+  float4 *cen = nullptr, *box = nullptr, *tmp_min = nullptr, *tmp_max = nullptr;
+  LbvhBufs lb;
+  int2 *chunk = nullptr, *pairs = nullptr;
+  int *leaf_cnt = nullptr, *leaf_off = nullptr;
+  int* counters = nullptr;  // [0] npairs [1] ncand_vf [2] ncand_ee [3] nvf [4] nee
+  void* scan_temp = nullptr;
+  NarrowBufs nb;
+  uint32_t* warn = nullptr;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = true;
+  // host copies
+  std::vector<float> inv_mass_h;
+  std::vector<uint8_t> pinned_h;
+  // counts of the host-built topology (for tests)
+  std::vector<int> h_sid, h_bid, h_prefix;
+};
+
+namespace {
+
+template <class T>
+int dalloc(cm_body* b, T** out, size_t count) {
+  *out = nullptr;
+  if (count == 0) count = 1;
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, sizeof(T) * count);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
+              " bytes): " + hipGetErrorString(e));
+    return CM_ERR_ALLOC;
+  }
+  b->allocs.push_back(ptr);
+  *out = (T*)ptr;
+  return CM_OK;
+}
+
+template <class T>
+int upload(cm_body* b, T** out, const std::vector<T>& h) {
+  int rc = dalloc(b, out, h.size());
+  if (rc) return rc;
+  if (!h.empty()) CM_HIP(hipMemcpy(*out, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return CM_OK;
+}
+
+float host_dist(const float* xyz, int a, int b) {
+  float dx = xyz[3 * a] - xyz[3 * b], dy = xyz[3 * a + 1] - xyz[3 * b + 1],
+        dz = xyz[3 * a + 2] - xyz[3 * b + 2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+float kp_of(float k, int iters) { return 1.f - std::pow(1.f - k, 1.f / (float)iters); }
+
+// the dihedral angle the reference computes at Physics/CudaPBD.cu:348-366
+float dihedral_rest(const float* xyz, int p1, int p2, int p3, int p4) {
+  auto d = [&](int a, int b) {
+    return mk3(xyz[3 * a] - xyz[3 * b], xyz[3 * a + 1] - xyz[3 * b + 1],
+               xyz[3 * a + 2] - xyz[3 * b + 2]);
+  };
+  f3 a = normalize(cross(d(p2, p1), d(p3, p1))), c = normalize(cross(d(p2, p1), d(p4, p1)));
+  // the dot of two unit vectors can round past +-1
+  return std::acos(fclamp(dot(a, c), -1.f, 1.f));
+}
+
+// Physics/CudaPBD.cu:289-346
+void quad_of(const int t1[3], const int t2[3], int q[4]) {
+  int p1 = -1, p2 = -1, p3 = -1, p4 = -1;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (t1[i] == t2[j]) {
+        p1 = t1[i];
+        break;
+      }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (t1[i] == t2[j] && t1[i] != p1) {
+        p2 = t1[i];
+        break;
+      }
+  for (int i = 0; i < 3 && p3 < 0; ++i)
+    if (t1[i] != p1 && t1[i] != p2) p3 = t1[i];
+  for (int i = 0; i < 3 && p4 < 0; ++i)
+    if (t2[i] != p1 && t2[i] != p2) p4 = t2[i];
+  q[0] = p1;
+  q[1] = p2;
+  q[2] = p3;
+  q[3] = p4;
+}
+
+inline uint64_t edge_key(int a, int b) {
+  uint32_t lo = (uint32_t)std::min(a, b), hi = (uint32_t)std::max(a, b);
+  return ((uint64_t)lo << 32) | hi;
+}
+
+struct HostTopo {
+  std::vector<int2> sid;
+  std::vector<float2> s_l0kp;
+  std::vector<int4> bid;
+  std::vector<float2> b_kpphi0;
+  std::vector<int> prefix;
+  std::vector<uint32_t> vslot;
+  std::vector<int4> tri;
+  std::vector<int> lra_fixed;
+  std::vector<float> lra_l0;
+};
+
+int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* tri, int t,
+                   const uint8_t* pinned, const int32_t* dist_pairs, int n_dist,
+                   const float* k_dist, const float* k_bend, HostTopo& h) {
+  const int I = P.solver_iterations;
+  std::unordered_map<uint64_t, int> open;
+  open.reserve((size_t)t * 2);
+  std::vector<int2> s_edges;
+  for (int i = 0; i < t; ++i) {
+    const int tv[3] = {(int)tri[3 * i], (int)tri[3 * i + 1], (int)tri[3 * i + 2]};
+    for (int j = 0; j < 3; ++j) {
+      const int id1 = tv[j % 3], id2 = tv[(j + 1) % 3];
+      const uint64_t k = edge_key(id1, id2);
+      auto it = open.find(k);
+      if (it != open.end()) {
+        const int f = it->second;
+        const int tf[3] = {(int)tri[3 * f], (int)tri[3 * f + 1], (int)tri[3 * f + 2]};
+        int q[4];
+        quad_of(tf, tv, q);
+        const int bi = (int)h.bid.size();
+        h.bid.push_back(make_int4(q[0], q[1], q[2], q[3]));
+        const float kb = k_bend ? k_bend[bi] : P.k_bend;
+        const float phi0 = (P.ref_compat & CM_RC_BEND_ABS) ? 3.14159265359f
+                                                           : dihedral_rest(xyz, q[0], q[1], q[2], q[3]);
+        h.b_kpphi0.push_back(make_float2(kp_of(kb, I), phi0));
+        open.erase(it);
+      } else {
+        open[k] = i;
+        s_edges.push_back(make_int2(id1, id2));
+      }
+    }
+  }
+  if (dist_pairs) {
+    s_edges.resize(n_dist);
+    for (int i = 0; i < n_dist; ++i) {
+      if (dist_pairs[2 * i] < 0 || dist_pairs[2 * i] >= n || dist_pairs[2 * i + 1] < 0 ||
+          dist_pairs[2 * i + 1] >= n) {
+        set_error("cm_create: distance constraint vertex out of range");
+        return CM_ERR_ARG;
+      }
+      s_edges[i] = make_int2(dist_pairs[2 * i], dist_pairs[2 * i + 1]);
+    }
+  }
+  const int S = (int)s_edges.size(), B = (int)h.bid.size();
+  if (S >= (1 << 29) || B >= (1 << 29)) {
+    set_error("cm_create: too many constraints");
+    return CM_ERR_ARG;
+  }
+  h.sid = s_edges;
+  h.s_l0kp.resize(S);
+  for (int i = 0; i < S; ++i)
+    h.s_l0kp[i] = make_float2(host_dist(xyz, s_edges[i].x, s_edges[i].y),
+                              kp_of(k_dist ? k_dist[i] : P.k_stretch, I));
+  // slots: every stretch end in constraint order, then every bend vertex
+  // (Physics/CudaPBD.cu:146-233)
+  std::vector<int> cnt(n, 0);
+  for (int i = 0; i < S; ++i) {
+    cnt[s_edges[i].x]++;
+    cnt[s_edges[i].y]++;
+  }
+  for (int i = 0; i < B; ++i) {
+    cnt[h.bid[i].x]++;
+    cnt[h.bid[i].y]++;
+    cnt[h.bid[i].z]++;
+    cnt[h.bid[i].w]++;
+  }
+  h.prefix.assign(n + 1, 0);
+  for (int i = 1; i <= n; ++i) h.prefix[i] = h.prefix[i - 1] + cnt[i - 1];
+  h.vslot.assign(h.prefix[n], 0);
+  std::fill(cnt.begin(), cnt.end(), 0);
+  for (int i = 0; i < S; ++i) {
+    const int a = s_edges[i].x, b = s_edges[i].y;
+    h.vslot[h.prefix[a] + cnt[a]++] = (uint32_t)i;
+    h.vslot[h.prefix[b] + cnt[b]++] = (uint32_t)i | (1u << 29);
+  }
+  for (int i = 0; i < B; ++i) {
+    const int q[4] = {h.bid[i].x, h.bid[i].y, h.bid[i].z, h.bid[i].w};
+    for (int r = 0; r < 4; ++r)
+      h.vslot[h.prefix[q[r]] + cnt[q[r]]++] = 0x80000000u | ((uint32_t)r << 29) | (uint32_t)i;
+  }
+  // feature ownership: the lowest triangle index claims each vertex / edge
+  std::vector<uint8_t> vown(n, 0);
+  std::unordered_map<uint64_t, int> eown;
+  eown.reserve((size_t)t * 2);
+  h.tri.resize(t);
+  for (int i = 0; i < t; ++i) {
+    const int a = (int)tri[3 * i], b = (int)tri[3 * i + 1], c = (int)tri[3 * i + 2];
+    int f = 0;
+    if (!vown[a]) { vown[a] = 1; f |= 1 << 5; }
+    if (!vown[b]) { vown[b] = 1; f |= 1 << 4; }
+    if (!vown[c]) { vown[c] = 1; f |= 1 << 3; }
+    if (eown.emplace(edge_key(a, b), 1).second) f |= 1 << 2;
+    if (eown.emplace(edge_key(b, c), 1).second) f |= 1 << 1;
+    if (eown.emplace(edge_key(c, a), 1).second) f |= 1 << 0;
+    h.tri[i] = make_int4(a, b, c, f);
+  }
+  // long-range attachments (Physics/CudaPBD.cu:60-81)
+  for (int i = 0; i < n; ++i)
+    if (pinned[i]) h.lra_fixed.push_back(i);
+  const int F = (int)h.lra_fixed.size();
+  h.lra_l0.assign((size_t)n * F, 0.f);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < F; ++j)
+      if (!pinned[i])
+        h.lra_l0[(size_t)i * F + j] = host_dist(xyz, i, h.lra_fixed[j]) * (1.f + P.strain_limit);
+  return CM_OK;
+}
+
+PbdBufs pbd_bufs(cm_body* b) {
+  PbdBufs pb;
+  pb.n = b->n;
+  pb.S = b->S;
+  pb.B = b->B;
+  pb.F = b->F;
+  pb.x = b->x;
+  pb.p = b->p;
+  pb.p2 = b->p2;
+  pb.v = b->v;
+  pb.pinned = b->pinned;
+  pb.sid = b->sid;
+  pb.s_l0kp = b->s_l0kp;
+  pb.bid = b->bid;
+  pb.b_kpphi0 = b->b_kpphi0;
+  pb.vprefix = b->vprefix;
+  pb.vslot = b->vslot;
+  pb.lra_fixed = b->lra_fixed;
+  pb.lra_l0 = b->lra_l0;
+  pb.lra_flag = b->lra_flag;
+  pb.ground = b->ground;
+  pb.mass = b->mass;
+  pb.damp_partials = b->damp_partials;
+  pb.damp_state = b->damp_state;
+  return pb;
+}
+
+PbdParams pbd_params(const cm_body* b) {
+  const cm_params& P = b->prm;
+  PbdParams pp;
+  pp.dt = P.timestep;
+  pp.th = P.thickness;
+  pp.gh = P.ground_height;
+  pp.g = mk3(P.gravity[0], P.gravity[1], P.gravity[2]);
+  pp.iters = P.solver_iterations;
+  pp.flags = P.flags;
+  pp.ref_compat = P.ref_compat;
+  pp.wmin = mk3(P.world_min[0], P.world_min[1], P.world_min[2]);
+  pp.wmax = mk3(P.world_max[0], P.world_max[1], P.world_max[2]);
+  pp.sc = mk3(P.sphere_center[0], P.sphere_center[1], P.sphere_center[2]);
+  pp.sr = P.sphere_radius;
+  pp.kg = P.global_vel_damp;
+  pp.kd = P.k_damp;
+  return pp;
+}
+
+// One substep on b->stream (Physics/CudaDeformableBody.cpp:550-627 stage order).
+int record_substep(cm_body* b) {
+  hipStream_t s = b->stream;
+  const cm_params& P = b->prm;
+  PbdBufs pb = pbd_bufs(b);
+  PbdParams pp = pbd_params(b);
+  int rc = launch_predict(pb, pp, s);
+  if (rc) return rc;
+  const bool coll = (P.flags & CM_F_SELF_COLLISION) && b->t >= 2;
+  if (coll) {
+    rc = launch_tri_update(b->n, b->t, b->x, b->p, b->tri, P.thickness, P.morton_bits,
+                           P.morton_normalize, b->lb.amin, b->lb.amax, b->keys, b->perm, b->cen,
+                           b->box, s);
+    if (rc) return rc;
+    rc = radix_sort(b->keys, b->perm, b->t, P.morton_bits == 30 ? 30 : 63, b->sort_temp, s);
+    if (rc) return rc;
+    rc = launch_gather_tris(b->t, b->perm, b->tri, b->tri_alt,
+                            (P.ref_compat & CM_RC_STALE_LEAF_AABB) ? 0 : 1, b->lb.amin,
+                            b->lb.amax, b->tmp_min, b->tmp_max, s);
+    if (rc) return rc;
+    rc = launch_build(b->lb, b->keys, (P.ref_compat & CM_RC_MIDPOINT_SPLIT) ? 1 : 0, s);
+    if (rc) return rc;
+    rc = launch_refit(b->lb, s);
+    if (rc) return rc;
+    rc = launch_pairs(b->lb, b->cap, b->chunk, b->leaf_cnt, b->leaf_off, b->pairs, b->counters,
+                      b->scan_temp, s);
+    if (rc) return rc;
+    rc = launch_narrow(b->nb, b->tri_alt, b->x, b->p, b->v, P.thickness, P.timestep,
+                       (P.flags & CM_F_EE_CONTACTS) ? 1 : 0, s);
+    if (rc) return rc;
+    rc = launch_dynamic_csr(b->nb, s);
+    if (rc) return rc;
+    rc = launch_copy_i4(b->t, b->tri_alt, b->tri, s);
+    if (rc) return rc;
+  }
+  NarrowBufs nb = b->nb;
+  if (!coll) nb.dprefix = nullptr;
+  rc = launch_solver(pb, pp, nb, s);
+  if (rc) return rc;
+  return launch_integrate(pb, pp, s);
+}
+
+int ensure_graph(cm_body* b) {
+  if (b->exec || !b->use_graph) return CM_OK;
+  CM_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = record_substep(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    return CM_ERR_HIP;
+  }
+  b->graph = g;
+  CM_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return CM_OK;
+}
+
+int body_check(cm_body* b) {
+  if (!b) {
+    set_error("null body");
+    return CM_ERR_ARG;
+  }
+  CM_HIP(hipSetDevice(b->dev));
+  return CM_OK;
+}
+
+int read_f4(cm_body* b, const float4* src, float* xyz) {
+  std::vector<float4> h(b->n);
+  CM_HIP(hipMemcpyAsync(h.data(), src, sizeof(float4) * b->n, hipMemcpyDeviceToHost, b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->n; ++i) {
+    xyz[3 * i] = h[i].x;
+    xyz[3 * i + 1] = h[i].y;
+    xyz[3 * i + 2] = h[i].z;
+  }
+  return CM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cm_last_error(void) { return g_err.c_str(); }
+
+// The reference's ClothParams literal (Managers/MastersSimulationManager.cpp:33-62) and
+// constants (Core/Utils.hpp:16-19, Physics/CudaDeformableUtils.cuh:16-17); wind off.
+void cm_default_params(cm_params* p) {
+  std::memset(p, 0, sizeof(*p));
+  p->timestep = 1.f / 60.f;
+  p->gravity[1] = -0.0981f;
+  p->k_stretch = 0.99f;
+  p->k_bend = 0.1f;
+  p->thickness = 0.f;
+  p->solver_iterations = 6;
+  p->ground_height = 0.f;
+  p->world_max[0] = p->world_max[1] = p->world_max[2] = 1.f;
+  p->global_vel_damp = 0.01f;
+  p->k_damp = 0.05f;
+  p->strain_limit = 0.15f;
+  p->k_friction = 0.1f;
+  p->pinned_mass = 1000.f;
+  p->morton_bits = 63;
+  p->morton_normalize = 0;
+  p->pair_cap = 64;
+  p->flags = CM_F_SELF_COLLISION | CM_F_GROUND | CM_F_WORLD_CLAMP | CM_F_DAMPING | CM_F_LRA |
+             CM_F_EE_CONTACTS | CM_F_BEND_FP64;
+  p->ref_compat = CM_RC_ALL;
+}
+
+int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32_t* tri,
+              int32_t t, const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
+              int32_t n_dist, const float* k_dist, const float* k_bend, cm_body** out) {
+  if (out) *out = nullptr;
+  if (!params || !xyz || n < 1 || !tri || t < 1 || !inv_mass || !out ||
+      params->solver_iterations < 1 || (params->morton_bits != 30 && params->morton_bits != 63) ||
+      params->pair_cap < 1 || (dist_pairs && n_dist < 0) || params->timestep <= 0.f) {
+    set_error("cm_create: invalid argument");
+    return CM_ERR_ARG;
+  }
+  for (int64_t i = 0; i < 3 * (int64_t)t; ++i)
+    if (tri[i] >= (uint32_t)n) {
+      set_error("cm_create: triangle index out of range");
+      return CM_ERR_ARG;
+    }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("cm_create: no HIP device");
+    return CM_ERR_NO_DEVICE;
+  }
+  cm_body* b = new cm_body();
+  b->prm = *params;
+  b->n = n;
+  b->t = t;
+  int rc = CM_OK;
+  auto fail = [&](int code) {
+    cm_destroy(b);
+    return code;
+  };
+  if (hipGetDevice(&b->dev) != hipSuccess) return fail(CM_ERR_HIP);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(CM_ERR_HIP);
+  }
+  b->pinned_h.resize(n);
+  b->inv_mass_h.assign(inv_mass, inv_mass + n);
+  for (int i = 0; i < n; ++i) b->pinned_h[i] = pinned ? (pinned[i] ? 1 : 0) : (inv_mass[i] == 0.f);
+  HostTopo h;
+  rc = build_topology(b->prm, xyz, n, tri, t, b->pinned_h.data(), dist_pairs, n_dist, k_dist,
+                      k_bend, h);
+  if (rc) return fail(rc);
+  b->S = (int)h.sid.size();
+  b->B = (int)h.bid.size();
+  b->F = (int)h.lra_fixed.size();
+  b->nslots = h.prefix[n];
+  for (auto& e : h.sid) {
+    b->h_sid.push_back(e.x);
+    b->h_sid.push_back(e.y);
+  }
+  for (auto& e : h.bid) {
+    b->h_bid.push_back(e.x);
+    b->h_bid.push_back(e.y);
+    b->h_bid.push_back(e.z);
+    b->h_bid.push_back(e.w);
+  }
+  b->h_prefix = h.prefix;
+  // state
+  std::vector<float4> x4(n), v4(n, make_float4(0, 0, 0, 0));
+  std::vector<float> mass(n);
+  for (int i = 0; i < n; ++i) {
+    x4[i] = make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], inv_mass[i]);
+    mass[i] = b->pinned_h[i] ? params->pinned_mass
+                             : (inv_mass[i] > 0 ? 1.f / inv_mass[i] : params->pinned_mass);
+  }
+  if ((rc = upload(b, &b->x, x4)) || (rc = upload(b, &b->p, x4)) || (rc = dalloc(b, &b->p2, n)) ||
+      (rc = upload(b, &b->v, v4)) || (rc = upload(b, &b->pinned, b->pinned_h)) ||
+      (rc = upload(b, &b->mass, mass)) || (rc = upload(b, &b->sid, h.sid)) ||
+      (rc = upload(b, &b->s_l0kp, h.s_l0kp)) || (rc = upload(b, &b->bid, h.bid)) ||
+      (rc = upload(b, &b->b_kpphi0, h.b_kpphi0)) || (rc = upload(b, &b->vprefix, h.prefix)) ||
+      (rc = upload(b, &b->vslot, h.vslot)) || (rc = upload(b, &b->lra_fixed, h.lra_fixed)) ||
+      (rc = upload(b, &b->lra_l0, h.lra_l0)) ||
+      (rc = dalloc(b, &b->lra_flag, (size_t)n * std::max(b->F, 1))) ||
+      (rc = dalloc(b, &b->ground, n)) ||
+      (rc = dalloc(b, &b->damp_partials, 9 * (size_t)nblocks(n))) ||
+      (rc = dalloc(b, &b->damp_state, 16)) || (rc = upload(b, &b->tri, h.tri)) ||
+      (rc = dalloc(b, &b->tri_alt, t)) || (rc = dalloc(b, &b->warn, 4)) ||
+      (rc = dalloc(b, &b->counters, 16)))
+    return fail(rc);
+  CM_HIP(hipMemset(b->warn, 0, 16));
+  CM_HIP(hipMemset(b->counters, 0, 64));
+  CM_HIP(hipMemset(b->ground, 0, sizeof(float4) * n));
+  // collision buffers
+  b->cap = std::min(params->pair_cap, t);
+  const size_t m = 2 * (size_t)t - 1;
+  const int64_t pcap = (int64_t)t * b->cap;
+  b->lb.t = t;
+  b->lb.warn = b->warn;
+  if ((rc = dalloc(b, &b->keys, t)) || (rc = dalloc(b, &b->perm, t)) ||
+      (rc = dalloc(b, (char**)&b->sort_temp, radix_sort_temp_bytes(t))) ||
+      (rc = dalloc(b, &b->cen, t)) || (rc = dalloc(b, &b->box, 2 * (size_t)nblocks(t) + 2)) ||
+      (rc = dalloc(b, &b->tmp_min, t)) || (rc = dalloc(b, &b->tmp_max, t)) ||
+      (rc = dalloc(b, &b->lb.amin, m)) || (rc = dalloc(b, &b->lb.amax, m)) ||
+      (rc = dalloc(b, &b->lb.lefts, t)) || (rc = dalloc(b, &b->lb.rights, t)) ||
+      (rc = dalloc(b, &b->lb.parents, m)) || (rc = dalloc(b, &b->lb.rmll, t)) ||
+      (rc = dalloc(b, &b->lb.rmlr, t)) || (rc = dalloc(b, &b->lb.claim_cnt, m)) ||
+      (rc = dalloc(b, &b->lb.claim_hi, m)) || (rc = dalloc(b, &b->lb.claim_lo, m)) ||
+      (rc = dalloc(b, &b->lb.visited, t)) || (rc = dalloc(b, &b->lb.done, t)) ||
+      (rc = dalloc(b, &b->chunk, pcap)) || (rc = dalloc(b, &b->pairs, pcap)) ||
+      (rc = dalloc(b, &b->leaf_cnt, t)) || (rc = dalloc(b, &b->leaf_off, t)) ||
+      (rc = dalloc(b, (char**)&b->scan_temp, scan_temp_bytes(0))))
+    return fail(rc);
+  CM_HIP(hipMemset(b->lb.parents, 0, sizeof(int) * m));
+  CM_HIP(hipMemset(b->lb.amin, 0, sizeof(float4) * m));
+  CM_HIP(hipMemset(b->lb.amax, 0, sizeof(float4) * m));
+  CM_HIP(hipMemset(b->lb.rmll, 0, sizeof(int) * t));
+  CM_HIP(hipMemset(b->lb.rmlr, 0, sizeof(int) * t));
+  CM_HIP(hipMemset(b->keys, 0, sizeof(uint64_t) * t));
+  CM_HIP(hipMemset(b->perm, 0, sizeof(uint32_t) * t));
+  // narrow phase: candidate capacities per triangle (reserved[0] / [1] override)
+  const int64_t vf_per = params->reserved[0] > 0 ? params->reserved[0] : 24;
+  const int64_t ee_per = params->reserved[1] > 0 ? params->reserved[1] : 48;
+  NarrowBufs& nb = b->nb;
+  nb.n = n;
+  nb.t = t;
+  nb.pair_cap = pcap;
+  nb.vf_cap = std::min<int64_t>(vf_per * t + 1024, 6 * pcap);
+  nb.ee_cap = std::min<int64_t>(ee_per * t + 1024, 9 * pcap);
+  nb.pairs = b->pairs;
+  nb.d_npairs = b->counters + 0;
+  nb.d_ncand_vf = b->counters + 1;
+  nb.d_ncand_ee = b->counters + 2;
+  nb.d_nvf = b->counters + 3;
+  nb.d_nee = b->counters + 4;
+  nb.d_nrec = b->counters + 5;
+  nb.warn = b->warn;
+  nb.scan_temp = b->scan_temp;
+  if ((rc = dalloc(b, &nb.mask, pcap)) || (rc = dalloc(b, &nb.cnt_vf, pcap)) ||
+      (rc = dalloc(b, &nb.cnt_ee, pcap)) || (rc = dalloc(b, &nb.cand_vf, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.cand_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.flag_vf, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.stage_w_vf, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.stage_tn_vf, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.stage_w_ee, nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.stage_tn_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.vf_v, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.vf_w, nb.vf_cap)) || (rc = dalloc(b, &nb.vf_tn, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.ee_v, nb.ee_cap)) || (rc = dalloc(b, &nb.ee_w, nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.ee_tn, nb.ee_cap)) || (rc = dalloc(b, &nb.dcount, n)) ||
+      (rc = dalloc(b, &nb.dprefix, (size_t)n + 1)) || (rc = dalloc(b, &nb.dcursor, n)) ||
+      (rc = dalloc(b, &nb.drec, 4 * (size_t)(nb.vf_cap + nb.ee_cap))))
+    return fail(rc);
+  CM_HIP(hipMemset(nb.dprefix, 0, sizeof(int) * ((size_t)n + 1)));
+  if (params->flags & CM_F_SPHERE) {
+    b->prm.flags |= CM_F_SPHERE;
+  }
+  CM_HIP(hipDeviceSynchronize());
+  *out = b;
+  return CM_OK;
+}
+
+int cm_destroy(cm_body* b) {
+  if (!b) return CM_OK;
+  hipSetDevice(b->dev);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (void* p : b->allocs) hipFree(p);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return CM_OK;
+}
+
+int cm_step(cm_body* b, int32_t substeps) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (substeps < 0) {
+    set_error("cm_step: negative substep count");
+    return CM_ERR_ARG;
+  }
+  rc = ensure_graph(b);
+  if (rc) return rc;
+  for (int s = 0; s < substeps; ++s) {
+    if (b->exec) {
+      CM_HIP(hipGraphLaunch(b->exec, b->stream));
+    } else {
+      rc = record_substep(b);
+      if (rc) return rc;
+    }
+  }
+  return CM_OK;
+}
+
+int cm_sync(cm_body* b) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  CM_HIP(hipStreamSynchronize(b->stream));
+  return CM_OK;
+}
+
+int cm_read_positions(cm_body* b, float* xyz) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (!xyz) {
+    set_error("cm_read_positions: null output");
+    return CM_ERR_ARG;
+  }
+  return read_f4(b, b->x, xyz);
+}
+
+int cm_read_velocities(cm_body* b, float* xyz) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (!xyz) {
+    set_error("cm_read_velocities: null output");
+    return CM_ERR_ARG;
+  }
+  return read_f4(b, b->v, xyz);
+}
+
+// Area-independent vertex normals: normalised sum of the unit face normals of
+// the incident triangles, computed on the host from the carried positions
+// (Physics/CudaDeformableUtils.cu:190-249 sums face normals with float atomics;
+// here the sum runs in triangle order).
+int cm_read_normals(cm_body* b, float* out) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (!out) {
+    set_error("cm_read_normals: null output");
+    return CM_ERR_ARG;
+  }
+  std::vector<float> xyz(3 * (size_t)b->n);
+  rc = read_f4(b, b->x, xyz.data());
+  if (rc) return rc;
+  std::vector<int4> tri(b->t);
+  CM_HIP(hipMemcpy(tri.data(), b->tri, sizeof(int4) * b->t, hipMemcpyDeviceToHost));
+  std::vector<f3> acc(b->n, mk3(0, 0, 0));
+  for (int i = 0; i < b->t; ++i) {
+    const int4 q = tri[i];
+    auto P = [&](int k) { return mk3(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]); };
+    const f3 nrm = normalize(cross(sub(P(q.y), P(q.x)), sub(P(q.z), P(q.x))));
+    acc[q.x] = add(acc[q.x], nrm);
+    acc[q.y] = add(acc[q.y], nrm);
+    acc[q.z] = add(acc[q.z], nrm);
+  }
+  for (int i = 0; i < b->n; ++i) {
+    const f3 nn = normalize(acc[i]);
+    out[3 * i] = nn.x;
+    out[3 * i + 1] = nn.y;
+    out[3 * i + 2] = nn.z;
+  }
+  return CM_OK;
+}
+
+int cm_set_sphere(cm_body* b, const float center[3], float radius) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  CM_HIP(hipStreamSynchronize(b->stream));
+  b->prm.sphere_center[0] = center[0];
+  b->prm.sphere_center[1] = center[1];
+  b->prm.sphere_center[2] = center[2];
+  b->prm.sphere_radius = radius;
+  b->prm.flags |= CM_F_SPHERE;
+  // kernel arguments changed: re-record the substep graph
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  return CM_OK;
+}
+
+int cm_set_state(cm_body* b, const float* x, const float* p, const float* v,
+                 const int32_t* tri) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (!x || !v) {
+    set_error("cm_set_state: x and v are required");
+    return CM_ERR_ARG;
+  }
+  std::vector<float4> hx(b->n), hp(b->n), hv(b->n);
+  const float* pp = p ? p : x;
+  for (int i = 0; i < b->n; ++i) {
+    const float w = b->inv_mass_h[i];
+    hx[i] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], w);
+    hp[i] = make_float4(pp[3 * i], pp[3 * i + 1], pp[3 * i + 2], w);
+    hv[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.f);
+  }
+  CM_HIP(hipStreamSynchronize(b->stream));
+  CM_HIP(hipMemcpy(b->x, hx.data(), sizeof(float4) * b->n, hipMemcpyHostToDevice));
+  CM_HIP(hipMemcpy(b->p, hp.data(), sizeof(float4) * b->n, hipMemcpyHostToDevice));
+  CM_HIP(hipMemcpy(b->v, hv.data(), sizeof(float4) * b->n, hipMemcpyHostToDevice));
+  if (tri) {
+    for (int64_t i = 0; i < 4 * (int64_t)b->t; i += 4)
+      if (tri[i] < 0 || tri[i] >= b->n || tri[i + 1] < 0 || tri[i + 1] >= b->n || tri[i + 2] < 0 ||
+          tri[i + 2] >= b->n) {
+        set_error("cm_set_state: triangle index out of range");
+        return CM_ERR_ARG;
+      }
+    CM_HIP(hipMemcpy(b->tri, tri, sizeof(int4) * b->t, hipMemcpyHostToDevice));
+  }
+  return CM_OK;
+}
+
+int cm_get_counts(cm_body* b, int64_t* counts) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  int c[8] = {0};
+  CM_HIP(hipMemcpyAsync(c, b->counters, sizeof(int) * 8, hipMemcpyDeviceToHost, b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
+  counts[0] = b->n;
+  counts[1] = b->t;
+  counts[2] = b->S;
+  counts[3] = b->B;
+  counts[4] = c[0];
+  counts[5] = c[3];
+  counts[6] = c[4];
+  return CM_OK;
+}
+
+int cm_get_warnings(cm_body* b, uint32_t* w) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  uint32_t h = 0;
+  CM_HIP(hipMemcpyAsync(&h, b->warn, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
+  *w = h & 0xffffu;
+  return CM_OK;
+}
+
+int cm_get_buffer(cm_body* b, int32_t which, void* host, int64_t capacity_bytes, int64_t* count) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  int c[8] = {0};
+  CM_HIP(hipMemcpyAsync(c, b->counters, sizeof(int) * 8, hipMemcpyDeviceToHost, b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
+  const void* src = nullptr;
+  size_t elem = 0;
+  int64_t n = 0;
+  const int t = b->t;
+  switch (which) {
+    case CM_BUF_X: src = b->x; elem = 16; n = b->n; break;
+    case CM_BUF_P: src = b->p; elem = 16; n = b->n; break;
+    case CM_BUF_V: src = b->v; elem = 16; n = b->n; break;
+    case CM_BUF_TRI: src = b->tri; elem = 16; n = t; break;
+    case CM_BUF_KEYS: src = b->keys; elem = 8; n = t; break;
+    case CM_BUF_PERM: src = b->perm; elem = 4; n = t; break;
+    case CM_BUF_LEFTS: src = b->lb.lefts; elem = 4; n = t - 1; break;
+    case CM_BUF_RIGHTS: src = b->lb.rights; elem = 4; n = t - 1; break;
+    case CM_BUF_PARENTS: src = b->lb.parents; elem = 4; n = 2 * (int64_t)t - 1; break;
+    case CM_BUF_RMLL: src = b->lb.rmll; elem = 4; n = t - 1; break;
+    case CM_BUF_RMLR: src = b->lb.rmlr; elem = 4; n = t - 1; break;
+    case CM_BUF_AABB_MIN: src = b->lb.amin; elem = 16; n = 2 * (int64_t)t - 1; break;
+    case CM_BUF_AABB_MAX: src = b->lb.amax; elem = 16; n = 2 * (int64_t)t - 1; break;
+    case CM_BUF_PAIRS: src = b->pairs; elem = 8; n = std::min<int64_t>(c[0], b->nb.pair_cap); break;
+    case CM_BUF_VF:
+    case CM_BUF_EE: {
+      const bool vf = which == CM_BUF_VF;
+      const int64_t cnt = vf ? c[3] : c[4];
+      if (count) *count = cnt;
+      if (!host) return CM_OK;
+      const int64_t k = std::min<int64_t>(cnt, capacity_bytes / (int64_t)sizeof(cm_contact));
+      std::vector<int4> hv(k);
+      std::vector<float4> hw(k), htn(k);
+      if (k > 0) {
+        CM_HIP(hipMemcpy(hv.data(), vf ? b->nb.vf_v : b->nb.ee_v, 16 * k, hipMemcpyDeviceToHost));
+        CM_HIP(hipMemcpy(hw.data(), vf ? b->nb.vf_w : b->nb.ee_w, 16 * k, hipMemcpyDeviceToHost));
+        CM_HIP(hipMemcpy(htn.data(), vf ? b->nb.vf_tn : b->nb.ee_tn, 16 * k, hipMemcpyDeviceToHost));
+      }
+      cm_contact* o = (cm_contact*)host;
+      for (int64_t i = 0; i < k; ++i) {
+        o[i].v[0] = hv[i].x; o[i].v[1] = hv[i].y; o[i].v[2] = hv[i].z; o[i].v[3] = hv[i].w;
+        o[i].w[0] = hw[i].x; o[i].w[1] = hw[i].y; o[i].w[2] = hw[i].z; o[i].w[3] = hw[i].w;
+        o[i].t = htn[i].x;
+        o[i].n[0] = htn[i].y; o[i].n[1] = htn[i].z; o[i].n[2] = htn[i].w;
+      }
+      return CM_OK;
+    }
+    default:
+      set_error("cm_get_buffer: unknown buffer");
+      return CM_ERR_ARG;
+  }
+  if (n < 0) n = 0;
+  if (count) *count = n;
+  if (host && n > 0) {
+    const int64_t k = std::min<int64_t>(n, capacity_bytes / (int64_t)elem);
+    CM_HIP(hipMemcpy(host, src, elem * k, hipMemcpyDeviceToHost));
+  }
+  return CM_OK;
+}
+
+int cm_time_kernel(cm_body* b, int32_t which, int32_t reps, float* avg_ms) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (reps < 1 || !avg_ms || (which != CM_KERNEL_SOLVE && which != CM_KERNEL_SUBSTEP)) {
+    set_error("cm_time_kernel: invalid argument");
+    return CM_ERR_ARG;
+  }
+  if (which == CM_KERNEL_SOLVE) reps += reps & 1;  // even: the result stays in p
+  rc = ensure_graph(b);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  CM_HIP(hipEventCreate(&e0));
+  CM_HIP(hipEventCreate(&e1));
+  CM_HIP(hipEventRecord(e0, b->stream));
+  if (which == CM_KERNEL_SOLVE) {
+    NarrowBufs nb = b->nb;
+    if (!((b->prm.flags & CM_F_SELF_COLLISION) && b->t >= 2)) nb.dprefix = nullptr;
+    rc = launch_solve_only(pbd_bufs(b), pbd_params(b), nb, reps, b->stream);
+  } else {
+    for (int r = 0; r < reps && !rc; ++r) {
+      if (b->exec) {
+        hipError_t e = hipGraphLaunch(b->exec, b->stream);
+        if (e != hipSuccess) {
+          set_error(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+          rc = CM_ERR_HIP;
+        }
+      } else {
+        rc = record_substep(b);
+      }
+    }
+  }
+  CM_HIP(hipEventRecord(e1, b->stream));
+  CM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *avg_ms = ms / (float)reps;
+  return CM_OK;
+}
+
+int cm_query_pairs(cm_body* b, int32_t* pairs, int32_t capacity, int32_t* n_pairs) {
+  int64_t cnt = 0;
+  int rc = cm_get_buffer(b, CM_BUF_PAIRS, pairs, (int64_t)capacity * 8, &cnt);
+  if (n_pairs) *n_pairs = (int32_t)cnt;
+  return rc;
+}
+
+int cm_query_contacts(cm_body* b, cm_contact* vf, int32_t cap_vf, int32_t* n_vf, cm_contact* ee,
+                      int32_t cap_ee, int32_t* n_ee) {
+  int64_t a = 0, c = 0;
+  int rc = cm_get_buffer(b, CM_BUF_VF, vf, (int64_t)cap_vf * sizeof(cm_contact), &a);
+  if (rc) return rc;
+  rc = cm_get_buffer(b, CM_BUF_EE, ee, (int64_t)cap_ee * sizeof(cm_contact), &c);
+  if (n_vf) *n_vf = (int32_t)a;
+  if (n_ee) *n_ee = (int32_t)c;
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+// Internal declarations of libclothmi: launch helpers shared between the
+// kernel translation units and the body orchestration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/clothmi.h"
+#include "cm_math.hpp"
+
+namespace cm {
+
+void set_error(const std::string& msg);
+
+#define CM_HIP(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      ::cm::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));              \
+      return CM_ERR_HIP;                                                               \
+    }                                                                                  \
+  } while (0)
+
+#define CM_LAUNCH_CHECK()                                                              \
+  do {                                                                                 \
+    hipError_t e_ = hipGetLastError();                                                 \
+    if (e_ != hipSuccess) {                                                            \
+      ::cm::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));         \
+      return CM_ERR_HIP;                                                               \
+    }                                                                                  \
+  } while (0)
+
+constexpr int kBlock = 256;  // 4 wave64s
+inline int nblocks(int64_t n, int block = kBlock) {
+  int64_t b = (n + block - 1) / block;
+  return (int)(b < 1 ? 1 : b);
+}
+inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
+
+// warnings raised by kernels (same bits as CM_WARN_*), plus internal bits
+constexpr uint32_t kRefitOverflow = 1u << 16;  // >2 parents on a node: refit fallback
+
+// ---- primitives (k_prims.hip) --------------------------------------------
+size_t scan_temp_bytes(int n_max);
+// exclusive prefix sum of in[0, n) into out[0, n), n = d_n ? min(*d_n, n_max) : n_max;
+// *d_total (if non-null) receives the sum. out may alias in.
+int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_total, void* temp,
+                   hipStream_t s);
+size_t radix_sort_temp_bytes(int n);
+int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s);
+int fill_i32(int* p, int v, int64_t n, hipStream_t s);
+int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+
+// ---- LBVH (k_lbvh.hip) -----------------------------------------------------
+struct LbvhBufs {
+  int t = 0;
+  float4 *amin = nullptr, *amax = nullptr;  // 2T-1 (internal nodes first)
+  int *lefts = nullptr, *rights = nullptr, *parents = nullptr, *rmll = nullptr, *rmlr = nullptr;
+  int *claim_cnt = nullptr, *claim_hi = nullptr, *claim_lo = nullptr;  // 2T-1
+  int* visited = nullptr;                                              // T-1
+  int* done = nullptr;                                                 // T-1 (fallback)
+  uint32_t* warn = nullptr;
+};
+size_t lbvh_build_temp_bytes(int t);
+size_t lbvh_pairs_temp_bytes(int t, int cap);
+int launch_tri_update(int n, int t, const float4* x, const float4* p, const int4* tri, float th,
+                      int bits, int normalize, float4* amin, float4* amax, uint64_t* keys,
+                      uint32_t* vals, float4* centroid_tmp, float4* box_tmp, hipStream_t s);
+int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStream_t s);
+int launch_refit(const LbvhBufs& b, hipStream_t s);
+// pairs: traversal into chunk (T*cap int2) + per-leaf counts, then compaction.
+int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
+                 int2* pairs, int* d_npairs, void* scan_temp, hipStream_t s);
+int launch_gather_tris(int t, const uint32_t* perm, const int4* tri_in, int4* tri_out,
+                       int fix_leaves, float4* amin, float4* amax, float4* tmp_min,
+                       float4* tmp_max, hipStream_t s);
+int launch_copy_i4(int64_t n, const int4* src, int4* dst, hipStream_t s);
+
+// ---- narrow phase (k_narrow.hip) ------------------------------------------
+struct NarrowBufs {
+  int n = 0, t = 0;
+  int64_t pair_cap = 0;       // capacity of pairs (T*cap)
+  int64_t vf_cap = 0, ee_cap = 0;
+  const int2* pairs = nullptr;
+  const int* d_npairs = nullptr;
+  uint16_t* mask = nullptr;   // per pair
+  int *cnt_vf = nullptr, *cnt_ee = nullptr;  // per pair (scanned in place)
+  int4 *cand_vf = nullptr, *cand_ee = nullptr;
+  int *d_ncand_vf = nullptr, *d_ncand_ee = nullptr;
+  int *flag_vf = nullptr, *flag_ee = nullptr;  // per candidate (scanned in place)
+  float4 *stage_w_vf = nullptr, *stage_tn_vf = nullptr, *stage_w_ee = nullptr,
+         *stage_tn_ee = nullptr;
+  int4 *vf_v = nullptr, *ee_v = nullptr;      // compacted contacts
+  float4 *vf_w = nullptr, *vf_tn = nullptr, *ee_w = nullptr, *ee_tn = nullptr;
+  int *d_nvf = nullptr, *d_nee = nullptr;
+  // per-vertex dynamic record CSR
+  int *dcount = nullptr, *dprefix = nullptr, *dcursor = nullptr;
+  uint32_t* drec = nullptr;  // 4 * (vf_cap + ee_cap)
+  int* d_nrec = nullptr;
+  uint32_t* warn = nullptr;
+  void* scan_temp = nullptr;
+};
+int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
+                  const float4* v, float th, float dt, int ee_on, hipStream_t s);
+int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s);
+
+// ---- PBD (k_pbd.hip) -------------------------------------------------------
+struct PbdBufs {
+  int n = 0, S = 0, B = 0, F = 0;
+  float4 *x = nullptr, *p = nullptr, *p2 = nullptr, *v = nullptr;  // .w of x/p = inverse mass
+  const uint8_t* pinned = nullptr;
+  const int2* sid = nullptr;
+  const float2* s_l0kp = nullptr;
+  const int4* bid = nullptr;
+  const float2* b_kpphi0 = nullptr;
+  const int* vprefix = nullptr;
+  const uint32_t* vslot = nullptr;
+  const int* lra_fixed = nullptr;
+  const float* lra_l0 = nullptr;
+  uint8_t* lra_flag = nullptr;
+  float4* ground = nullptr;  // qc.xyz, flag
+  const float* mass = nullptr;
+  float* damp_partials = nullptr;  // per block sums
+  float* damp_state = nullptr;     // omega, xcm, vcm
+};
+struct PbdParams {
+  float dt, th, gh;
+  f3 g;
+  int iters;
+  uint32_t flags, ref_compat;
+  f3 wmin, wmax;
+  f3 sc;
+  float sr;
+  float kg, kd;
+};
+int launch_predict(const PbdBufs& b, const PbdParams& pp, hipStream_t s);
+int launch_solver(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, hipStream_t s);
+int launch_solve_only(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, int iters,
+                      hipStream_t s);
+int launch_integrate(const PbdBufs& b, const PbdParams& pp, hipStream_t s);
+
+}  // namespace cm

@@ -1,0 +1,482 @@
+// Narrow phase of the substep: candidate expansion, feature swept-AABB filter,
+// continuous collision tests and ordered compaction, plus the per-vertex record
+// lists the solver gathers contact corrections from.
+//
+//  k_cand_count / k_cand_write <- DeformableUtils::CreateTriangleTests
+//        (LicentaV2/LicentaV2/Collision/CudaCollisionDetection.cu:6-189) fused with
+//        CudaPrimitiveTests::FeatureAABBTests (Collision/CudaPrimitiveTests.cu:427-490);
+//        vertex and edge swept AABBs are recomputed from p and x on the fly
+//        (same values as Collision/FeatureList.cu:134-162), which removes the
+//        reference's N x N edge map.
+//  k_ccd <- CudaPrimitiveTests::CCDTriangleTests (Collision/CudaPrimitiveTests.cu:109-238)
+//        with Physics::FindCoplanarityTimes (Physics/CudaCubicSolver.cu:5-153)
+//  compaction keeps candidate order, like CubWrap::SelectFlagged (Core/CubWrappers.cuh:24-50)
+//  dynamic CSR: records (contact, role) grouped by vertex in record order, the order
+//        the reference's stable sort + run-length encode gives
+//        (Physics/CudaPBD.cu:669-725, 1448-1475).
+#include "cm_internal.hpp"
+
+namespace cm {
+
+namespace {
+
+__device__ __forceinline__ int bit(int f, int b) { return (f >> b) & 1; }
+
+__device__ __forceinline__ void vert_aabb(const float4* __restrict__ p,
+                                          const float4* __restrict__ x, int i, float th, f3& mn,
+                                          f3& mx) {
+  const f3 a = xyz(p[i]), b = xyz(x[i]);
+  mn = subs(vmin(a, b), th);
+  mx = adds(vmax(a, b), th);
+}
+
+// EE slot table: (edge of tri1, edge of tri2) as vertex index pairs into {v1,v2,v3}
+// and the ownership bits that gate them (Collision/CudaCollisionDetection.cu:139-187).
+__constant__ int kEE1[9][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 1}, {0, 2}, {1, 2}, {0, 1}, {0, 2}, {1, 2}};
+__constant__ int kEE2[9][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 2}, {0, 2}, {0, 2}, {1, 2}, {1, 2}, {1, 2}};
+__constant__ int kEEB1[9] = {2, 0, 1, 2, 0, 1, 2, 0, 1};
+__constant__ int kEEB2[9] = {2, 2, 2, 0, 0, 0, 1, 1, 1};
+
+__global__ void __launch_bounds__(kBlock) k_cand_count(
+    const int2* __restrict__ pairs, const int* __restrict__ d_npairs, int64_t pair_cap,
+    const int4* __restrict__ tri, const float4* __restrict__ x, const float4* __restrict__ p,
+    float th, int ee_on, uint16_t* __restrict__ mask, int* __restrict__ cnt_vf,
+    int* __restrict__ cnt_ee) {
+  const int np = (int)min((int64_t)*d_npairs, pair_cap);
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < np; k += gridDim.x * kBlock) {
+    const int2 pr = pairs[k];
+    const int4 t1 = tri[pr.x], t2 = tri[pr.y];
+    const int v1[3] = {t1.x, t1.y, t1.z}, v2[3] = {t2.x, t2.y, t2.z};
+    f3 mn1[3], mx1[3], mn2[3], mx2[3];
+    for (int a = 0; a < 3; ++a) {
+      vert_aabb(p, x, v1[a], th, mn1[a], mx1[a]);
+      vert_aabb(p, x, v2[a], th, mn2[a], mx2[a]);
+    }
+    unsigned m = 0;
+    // VF: vertices of tri1 against tri2, then vertices of tri2 against tri1
+    const f3 tmn2 = min3(mn2[0], mn2[1], mn2[2]), tmx2 = max3(mx2[0], mx2[1], mx2[2]);
+    const f3 tmn1 = min3(mn1[0], mn1[1], mn1[2]), tmx1 = max3(mx1[0], mx1[1], mx1[2]);
+    for (int s = 0; s < 3; ++s) {
+      if (bit(t1.w, 5 - s) && overlap(mn1[s], mx1[s], tmn2, tmx2)) m |= 1u << s;
+      if (bit(t2.w, 5 - s) && overlap(mn2[s], mx2[s], tmn1, tmx1)) m |= 1u << (3 + s);
+    }
+    if (ee_on) {
+      for (int s = 0; s < 9; ++s) {
+        if (!(bit(t1.w, kEEB1[s]) & bit(t2.w, kEEB2[s]))) continue;
+        const int a0 = kEE1[s][0], a1 = kEE1[s][1], b0 = kEE2[s][0], b1 = kEE2[s][1];
+        const f3 amn = vmin(mn1[a0], mn1[a1]), amx = vmax(mx1[a0], mx1[a1]);
+        const f3 bmn = vmin(mn2[b0], mn2[b1]), bmx = vmax(mx2[b0], mx2[b1]);
+        if (overlap(amn, amx, bmn, bmx)) m |= 1u << (6 + s);
+      }
+    }
+    mask[k] = (uint16_t)m;
+    cnt_vf[k] = __popc(m & 63u);
+    cnt_ee[k] = __popc(m >> 6);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cand_write(
+    const int2* __restrict__ pairs, const int* __restrict__ d_npairs, int64_t pair_cap,
+    const int4* __restrict__ tri, const uint16_t* __restrict__ mask,
+    const int* __restrict__ off_vf, const int* __restrict__ off_ee, int4* __restrict__ cand_vf,
+    int4* __restrict__ cand_ee, int64_t vf_cap, int64_t ee_cap, uint32_t* __restrict__ warn) {
+  const int np = (int)min((int64_t)*d_npairs, pair_cap);
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < np; k += gridDim.x * kBlock) {
+    const unsigned m = mask[k];
+    if (!m) continue;
+    const int2 pr = pairs[k];
+    const int4 t1 = tri[pr.x], t2 = tri[pr.y];
+    const int v1[3] = {t1.x, t1.y, t1.z}, v2[3] = {t2.x, t2.y, t2.z};
+    int o = off_vf[k];
+    bool over = false;
+    for (int s = 0; s < 6; ++s) {
+      if (!(m & (1u << s))) continue;
+      const int4 q = s < 3 ? make_int4(v1[s], t2.x, t2.y, t2.z)
+                           : make_int4(v2[s - 3], t1.x, t1.y, t1.z);
+      if (o < vf_cap) cand_vf[o] = q; else over = true;
+      ++o;
+    }
+    o = off_ee[k];
+    for (int s = 0; s < 9; ++s) {
+      if (!(m & (1u << (6 + s)))) continue;
+      const int4 q = make_int4(v1[kEE1[s][0]], v1[kEE1[s][1]], v2[kEE2[s][0]], v2[kEE2[s][1]]);
+      if (o < ee_cap) cand_ee[o] = q; else over = true;
+      ++o;
+    }
+    if (over) atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
+  }
+}
+
+// ---- CCD (FP32, same arithmetic as the oracle) -------------------------------
+constexpr float kCubicEps = 1.f / 1000000;  // Physics/CudaCubicSolver.cuh:12
+constexpr float kUtilEps = 0.00000001f;     // Core/CudaUtils.cu:4
+
+__device__ __forceinline__ float triple_scalar(f3 x21, f3 x31, f3 x41, f3 v21, f3 v31, f3 v41,
+                                               float t) {
+  return dot(cross(add(x21, mul(v21, t)), add(x31, mul(v31, t))), add(x41, mul(v41, t)));
+}
+
+__device__ float secant(f3 x21, f3 x31, f3 x41, f3 v21, f3 v31, f3 v41, float a, float b) {
+  float x0 = a, x1 = b;
+  for (int i = 0; i < 20; ++i) {
+    const float f0 = triple_scalar(x21, x31, x41, v21, v31, v41, x0);
+    const float f1 = triple_scalar(x21, x31, x41, v21, v31, v41, x1);
+    const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
+    if (fabsf(xc - x1) < kCubicEps) return xc;
+    x0 = x1;
+    x1 = xc;
+  }
+  return -15.f;
+}
+
+__device__ int coplanarity_times(f3 x1, f3 x2, f3 x3, f3 x4, f3 v1, f3 v2, f3 v3, f3 v4,
+                                 float* times) {
+  int n = 0;
+  const f3 x21 = sub(x2, x1), x31 = sub(x3, x1), x41 = sub(x4, x1);
+  const f3 v21 = sub(v2, v1), v31 = sub(v3, v1), v41 = sub(v4, v1);
+  const f3 v21v31 = cross(v21, v31), x21v31 = cross(x21, v31), v21x31 = cross(v21, x31),
+           x21x31 = cross(x21, x31);
+  const float v1v2v3 = dot(v21v31, v41), x1v2v3 = dot(x21v31, v41), v1x2v3 = dot(v21x31, v41),
+              v1v2x3 = dot(v21v31, x41), x1x2v3 = dot(x21x31, v41), x1v2x3 = dot(x21v31, x41),
+              v1x2x3 = dot(v21x31, x41);
+  const float a = v1v2v3;
+  const float b = x1v2v3 + v1x2v3 + v1v2x3;
+  const float c = x1x2v3 + x1v2x3 + v1x2x3;
+  const float ad = 3 * a, bd = 2 * b;
+  const float dd = 4 * (b * b - ad * c);
+  const float sq = sqrtf(dd);
+  const float da = 2 * ad;
+  const float t1 = ((-bd + sq) / da);
+  const float t2 = ((-bd - sq) / da);
+  float cp1 = 0.f, cp2 = 0.f;
+  int cc = 0;
+  bool have1 = false;
+  if (dd >= kCubicEps) {
+    if (between(t1, 0.f, 1.f)) { cp1 = t1; have1 = true; ++cc; }
+    if (between(t2, 0.f, 1.f)) { cp2 = t2; ++cc; }
+  } else if (fabsf(dd) < kCubicEps) {
+    if (between(t1, 0.f, 1.f)) { cp1 = t1; have1 = true; ++cc; }
+  }
+  if (cc == 1 && !have1) cp1 = cp2;
+  float ends[4];
+  int ne = 0;
+  ends[ne++] = 0.f;
+  if (cc >= 1) ends[ne++] = cp1;
+  if (cc == 2) ends[ne++] = cp2;
+  ends[ne++] = 1.f;
+  for (int i = 0; i < ne - 1; ++i) {
+    const float ea = ends[i], eb = ends[i + 1];
+    const float fa = triple_scalar(x21, x31, x41, v21, v31, v41, ea);
+    const float fb = triple_scalar(x21, x31, x41, v21, v31, v41, eb);
+    if (fabsf(fa) < kCubicEps) times[n++] = ea;
+    if (fa * fb < 0.f) {
+      const float tt = secant(x21, x31, x41, v21, v31, v41, ea, eb);
+      if (tt != -15) times[n++] = tt;
+    }
+  }
+  return n;
+}
+
+struct Hit {
+  float w[4];
+  float t;
+  f3 n;
+};
+
+__device__ bool test_vf(f3 x4, f3 x1, f3 x2, f3 x3, Hit& h, float th) {
+  const f3 x13 = sub(x1, x3), x23 = sub(x2, x3), x43 = sub(x4, x3);
+  const f3 crs = cross(sub(x2, x1), sub(x3, x1));
+  const f3 n = normalize(crs);
+  if (fabsf(dot(x43, n)) <= th) {
+    const float dtmp = dot(x13, x23);
+    const float m11 = dot(x13, x13), m12 = dtmp, m21 = dtmp, m22 = dot(x23, x23);
+    const float vr1 = dot(x13, x43), vr2 = dot(x23, x43);
+    const float w2 = (m11 * vr2 - m21 * vr1) / (m11 * m22 - m21 * m12);
+    const float w1 = (vr1 - m12 * w2) / m11;
+    const float w3 = 1.f - w1 - w2;
+    const float delta = th / (len(x13) + len(x23) + len(sub(x1, x2)));
+    const float lo = 0.f, hi = 1 + delta;
+    if (between(w1, lo, hi) && between(w2, lo, hi) && between(w3, lo, hi)) {
+      h.w[0] = 1.f;
+      h.w[1] = -w1;
+      h.w[2] = -w2;
+      h.w[3] = -w3;
+      h.n = n;
+      h.t = 0.f;
+      return true;
+    }
+  }
+  return false;
+}
+
+__device__ f3 project_on_line(f3 p, f3 a, f3 b) {
+  const f3 ab = sub(b, a), ap = sub(p, a);
+  return add(a, mul(ab, dot(ap, ab) / dot(ab, ab)));
+}
+__device__ f3 clamp_on_line(f3 p, f3 a, f3 b) {
+  const float d1 = len(sub(p, a)), d2 = len(sub(p, b));
+  if (fabsf(d1 + d2 - len(sub(a, b))) < kUtilEps) return p;
+  return d1 < d2 ? a : b;
+}
+
+__device__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, float th) {
+  const f3 x21 = sub(x2, x1), x31 = sub(x3, x1), x43 = sub(x4, x3);
+  const f3 crs = cross(x21, x43);
+  if (len(crs) < 0.000000001f) {  // TestEdgeDegenerate (Collision/CudaPrimitiveTests.cu:408-425)
+    const f3 m1 = add(x1, divs(x21, 2.f)), m2 = add(x3, divs(x43, 2.f));
+    if (len(sub(m1, m2)) <= th) {
+      h.w[0] = h.w[1] = h.w[2] = h.w[3] = 0.5f;
+      h.n = normalize(sub(m1, m2));
+      return true;
+    }
+    return false;
+  }
+  if (is_zero(crs)) return false;
+  const float dtmp = dot(neg(x21), x43);
+  const float m11 = dot(x21, x21), m12 = dtmp, m21 = dtmp, m22 = dot(x43, x43);
+  const float vr1 = dot(x21, x31), vr2 = dot(neg(x43), x31);
+  float b = (m11 * vr2 - m21 * vr1) / (m11 * m22 - m21 * m12);
+  float a = (vr1 - m12 * b) / m11;
+  if (between(a, 0.f, 1.f) && between(b, 0.f, 1.f)) {
+    const f3 cp1 = add(x1, mul(x21, a));
+    const f3 cp2 = add(x3, mul(x43, a));
+    if (len(sub(cp1, cp2)) <= th) {
+      h.w[0] = 1 - a;
+      h.w[1] = a;
+      h.w[2] = -(1 - b);
+      h.w[3] = -b;
+      const f3 dir = sub(add(x1, mul(x21, a)), add(x3, mul(x43, b)));
+      h.n = normalize(is_zero(dir) ? crs : dir);
+      return true;
+    }
+  } else {
+    const float ac = fclamp(a, 0.f, 1.f), bc = fclamp(b, 0.f, 1.f);
+    const float d1 = fabsf(a - ac) * len(x21);
+    const float d2 = fabsf(b - bc) * len(x43);
+    a = ac;
+    b = bc;
+    f3 cp1, cp2;
+    if (d1 > d2) {
+      cp1 = add(x1, mul(x21, a));
+      cp2 = clamp_on_line(project_on_line(cp1, x3, x4), x3, x4);
+    } else {
+      cp2 = add(x3, mul(x43, b));
+      cp1 = clamp_on_line(project_on_line(cp2, x1, x2), x1, x2);
+    }
+    if (len(sub(cp1, cp2)) <= th) {
+      h.w[0] = 1 - a;
+      h.w[1] = a;
+      h.w[2] = -(1 - b);
+      h.w[3] = -b;
+      h.n = normalize(sub(cp1, cp2));
+      h.t = 0.f;
+      return true;
+    }
+  }
+  return false;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ccd(
+    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
+    const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
+    int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
+    float dt, int* __restrict__ flag_vf, int* __restrict__ flag_ee, float4* __restrict__ w_vf,
+    float4* __restrict__ tn_vf, float4* __restrict__ w_ee, float4* __restrict__ tn_ee) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < nvf + nee; id += gridDim.x * kBlock) {
+    const bool isvf = id < nvf;
+    const int k = isvf ? id : id - nvf;
+    const int4 c = isvf ? cand_vf[k] : cand_ee[k];
+    bool hit = false;
+    Hit h;
+    h.t = -1.f;
+    h.w[0] = h.w[1] = h.w[2] = h.w[3] = -1.f;
+    h.n = mk3(-1.f, -1.f, -1.f);
+    float times[10];
+    if (isvf) {
+      if (!(c.x == c.y || c.x == c.z || c.x == c.w)) {  // VFSanity
+        f3 x4 = xyz(x[c.x]), x1 = xyz(x[c.y]), x2 = xyz(x[c.z]), x3 = xyz(x[c.w]);
+        const f3 v4 = xyz(v[c.x]), v1 = xyz(v[c.y]), v2 = xyz(v[c.z]), v3 = xyz(v[c.w]);
+        const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
+        for (int i = 0; i < nt && !hit; ++i) {
+          const float t = times[i] * dt;
+          x4 = add(x4, mul(v4, t));
+          x1 = add(x1, mul(v1, t));
+          x2 = add(x2, mul(v2, t));
+          x3 = add(x3, mul(v3, t));
+          if (test_vf(x4, x1, x2, x3, h, th)) {
+            h.t = t;
+            hit = true;
+          }
+        }
+      }
+      flag_vf[k] = hit;
+      if (hit) {
+        w_vf[k] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+        tn_vf[k] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+      }
+    } else {
+      if (!(c.x == c.z || c.x == c.w || c.y == c.z || c.y == c.w)) {  // EESanity
+        f3 x1 = xyz(x[c.x]), x2 = xyz(x[c.y]), x3 = xyz(x[c.z]), x4 = xyz(x[c.w]);
+        const f3 v1 = xyz(v[c.x]), v2 = xyz(v[c.y]), v3 = xyz(v[c.z]), v4 = xyz(v[c.w]);
+        const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
+        for (int i = 0; i < nt && !hit; ++i) {
+          const float t = times[i] * dt;
+          x1 = add(x1, mul(v1, t));
+          x2 = add(x2, mul(v2, t));
+          x3 = add(x3, mul(v3, t));
+          x4 = add(x4, mul(v4, t));
+          if (test_ee(x1, x2, x3, x4, h, th)) {
+            h.t = t;
+            hit = true;
+          }
+        }
+      }
+      flag_ee[k] = hit;
+      if (hit) {
+        w_ee[k] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+        tn_ee[k] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+      }
+    }
+  }
+}
+
+// flags were scanned in place into offsets; total in *d_n. Recover each flag as
+// off[k+1]-off[k] (the last one from the total).
+__global__ void __launch_bounds__(kBlock) k_compact_contacts(
+    const int4* __restrict__ cand, const int* __restrict__ d_ncand, int64_t cap,
+    const int* __restrict__ off, const int* __restrict__ d_n, const float4* __restrict__ sw,
+    const float4* __restrict__ stn, int4* __restrict__ ov, float4* __restrict__ ow,
+    float4* __restrict__ otn) {
+  const int nc = (int)min((int64_t)*d_ncand, cap);
+  const int total = *d_n;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nc; k += gridDim.x * kBlock) {
+    const int o = off[k];
+    const int next = k + 1 < nc ? off[k + 1] : total;
+    if (next > o) {
+      ov[o] = cand[k];
+      ow[o] = sw[k];
+      otn[o] = stn[k];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_dyn_count(const int4* __restrict__ vfv,
+                                                      const int4* __restrict__ eev,
+                                                      const int* __restrict__ d_nvf,
+                                                      const int* __restrict__ d_nee,
+                                                      int* __restrict__ dcount) {
+  const int nvf = *d_nvf, nee = *d_nee;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
+    const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
+    atomicAdd(&dcount[q.x], 1);
+    atomicAdd(&dcount[q.y], 1);
+    atomicAdd(&dcount[q.z], 1);
+    atomicAdd(&dcount[q.w], 1);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_dyn_fill(const int4* __restrict__ vfv,
+                                                     const int4* __restrict__ eev,
+                                                     const int* __restrict__ d_nvf,
+                                                     const int* __restrict__ d_nee,
+                                                     int* __restrict__ cursor,
+                                                     uint32_t* __restrict__ drec) {
+  const int nvf = *d_nvf, nee = *d_nee;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
+    const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
+    const int vv[4] = {q.x, q.y, q.z, q.w};
+    for (int r = 0; r < 4; ++r) drec[atomicAdd(&cursor[vv[r]], 1)] = 4u * (uint32_t)c + r;
+  }
+}
+
+// per vertex: order its records by record index (insertion sort; lists are short)
+__global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restrict__ prefix,
+                                                     uint32_t* __restrict__ drec) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= n) return;
+  const int b = prefix[v], e = prefix[v + 1];
+  for (int i = b + 1; i < e; ++i) {
+    const uint32_t key = drec[i];
+    int j = i - 1;
+    while (j >= b && drec[j] > key) {
+      drec[j + 1] = drec[j];
+      --j;
+    }
+    drec[j + 1] = key;
+  }
+}
+
+__global__ void k_copy_i32(const int* __restrict__ s, int* __restrict__ d, int n) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) d[i] = s[i];
+}
+
+__global__ void k_min_cap(int* __restrict__ d_n, int64_t cap) {
+  if (threadIdx.x == 0 && (int64_t)*d_n > cap) *d_n = (int)cap;
+}
+
+}  // namespace
+
+static int grid_for(int64_t cap) {
+  int64_t g = (cap + kBlock - 1) / kBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
+int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
+                  const float4* v, float th, float dt, int ee_on, hipStream_t s) {
+  const int gp = grid_for(nb.pair_cap);
+  hipLaunchKernelGGL(k_cand_count, dim3(gp), dim3(kBlock), 0, s, nb.pairs, nb.d_npairs,
+                     nb.pair_cap, tri, x, p, th, ee_on, nb.mask, nb.cnt_vf, nb.cnt_ee);
+  CM_LAUNCH_CHECK();
+  int rc = scan_exclusive(nb.cnt_vf, nb.cnt_vf, (int)nb.pair_cap, nb.d_npairs, nb.d_ncand_vf,
+                          nb.scan_temp, s);
+  if (rc) return rc;
+  rc = scan_exclusive(nb.cnt_ee, nb.cnt_ee, (int)nb.pair_cap, nb.d_npairs, nb.d_ncand_ee,
+                      nb.scan_temp, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_cand_write, dim3(gp), dim3(kBlock), 0, s, nb.pairs, nb.d_npairs,
+                     nb.pair_cap, tri, nb.mask, nb.cnt_vf, nb.cnt_ee, nb.cand_vf, nb.cand_ee,
+                     nb.vf_cap, nb.ee_cap, nb.warn);
+  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_vf, nb.vf_cap);
+  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_ee, nb.ee_cap);
+  hipLaunchKernelGGL(k_ccd, dim3(grid_for(nb.vf_cap + nb.ee_cap)), dim3(kBlock), 0, s, nb.cand_vf,
+                     nb.cand_ee, nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, th, dt,
+                     nb.flag_vf, nb.flag_ee, nb.stage_w_vf, nb.stage_tn_vf, nb.stage_w_ee,
+                     nb.stage_tn_ee);
+  CM_LAUNCH_CHECK();
+  rc = scan_exclusive(nb.flag_vf, nb.flag_vf, (int)nb.vf_cap, nb.d_ncand_vf, nb.d_nvf,
+                      nb.scan_temp, s);
+  if (rc) return rc;
+  rc = scan_exclusive(nb.flag_ee, nb.flag_ee, (int)nb.ee_cap, nb.d_ncand_ee, nb.d_nee,
+                      nb.scan_temp, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_compact_contacts, dim3(grid_for(nb.vf_cap)), dim3(kBlock), 0, s,
+                     nb.cand_vf, nb.d_ncand_vf, nb.vf_cap, nb.flag_vf, nb.d_nvf, nb.stage_w_vf,
+                     nb.stage_tn_vf, nb.vf_v, nb.vf_w, nb.vf_tn);
+  hipLaunchKernelGGL(k_compact_contacts, dim3(grid_for(nb.ee_cap)), dim3(kBlock), 0, s,
+                     nb.cand_ee, nb.d_ncand_ee, nb.ee_cap, nb.flag_ee, nb.d_nee, nb.stage_w_ee,
+                     nb.stage_tn_ee, nb.ee_v, nb.ee_w, nb.ee_tn);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
+  int rc = fill_i32(nb.dcount, 0, nb.n, s);
+  if (rc) return rc;
+  const int gc = grid_for(nb.vf_cap + nb.ee_cap);
+  hipLaunchKernelGGL(k_dyn_count, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
+                     nb.d_nee, nb.dcount);
+  CM_LAUNCH_CHECK();
+  rc = scan_exclusive(nb.dcount, nb.dprefix, nb.n, nullptr, nb.dprefix + nb.n, nb.scan_temp, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_copy_i32, dim3(grid_for(nb.n)), dim3(kBlock), 0, s, nb.dprefix, nb.dcursor,
+                     nb.n);
+  hipLaunchKernelGGL(k_dyn_fill, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
+                     nb.d_nee, nb.dcursor, nb.drec);
+  hipLaunchKernelGGL(k_dyn_sort, dim3(nblocks(nb.n)), dim3(kBlock), 0, s, nb.n, nb.dprefix,
+                     nb.drec);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+}  // namespace cm

@@ -1,0 +1,562 @@
+// Position-based-dynamics stages of the substep as gfx950 kernels.
+//
+//  k_predict      <- CudaPBD::ApplyExternalForces + ExplicitEuler
+//                    (LicentaV2/LicentaV2/Physics/CudaPBD.cu:1992-2012, 610-630)
+//  k_damp_*       <- CudaPBD::DampVelocities (Physics/CudaPBD.cu:411-458, 1934-1990);
+//                    sums are block trees over 256 consecutive vertices, block
+//                    results added in block order (a fixed, reproducible order)
+//  k_setup        <- _CreateGroundConstraints / _CreateLRAConstraints (:1477-1507, :1571-1592)
+//  k_sphere       <- _CreateSphereConstraints + RaySphere (:1509-1569,
+//                    Collision/CudaCollisionDetection.cu:529-567)
+//  k_solve        <- one Jacobi iteration of PBDStepSolver (:648-659): ground, LRA,
+//                    static (stretch :986-1022, bend :1024-1147), dynamic (VF :1212-1311,
+//                    EE :1313-1415) corrections and FinalCorrectionStep (:1690-1698),
+//                    fused per vertex: each vertex re-evaluates the constraints it
+//                    belongs to in the reference's slot order, so the per-vertex sums
+//                    are the reference's gathers (:1417-1475) without the slot buffer.
+//  k_integrate    <- _FinalUpdate (:1916-1932) + _WorldConstraints
+//                    (Physics/CudaDeformableUtils.cu:260-269)
+//
+// Positions are float4 with the inverse mass in .w, so one 16-byte load gives a
+// neighbour's position and weight.
+#include "cm_internal.hpp"
+
+namespace cm {
+
+namespace {
+
+__global__ void __launch_bounds__(kBlock) k_predict(int n, const float4* __restrict__ x,
+                                                    float4* __restrict__ v,
+                                                    float4* __restrict__ p, float dt, f3 g) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 xi = x[i];
+  const f3 dtg = mul(g, dt);
+  const f3 vi = add(xyz(v[i]), dtg);
+  v[i] = f4(vi, 0.f);
+  p[i] = f4(add(xyz(xi), mul(vi, dt)), xi.w);
+}
+
+// ---- damping ------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void block_tree(float (&vals)[K], float (*s)[kBlock], float* out) {
+  for (int k = 0; k < K; ++k) s[k][threadIdx.x] = vals[k];
+  __syncthreads();
+  for (int stride = kBlock / 2; stride >= 1; stride >>= 1) {
+    if (threadIdx.x < stride)
+      for (int k = 0; k < K; ++k) s[k][threadIdx.x] = s[k][threadIdx.x] + s[k][threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; ++k) out[k] = s[k][0];
+}
+
+// v' = (v + dt g)(1 - kg); partial sums of m, p m, v' m
+__global__ void __launch_bounds__(kBlock) k_damp_sum1(int n, const float4* __restrict__ p,
+                                                      float4* __restrict__ v,
+                                                      const float* __restrict__ mass, f3 dtg,
+                                                      float kg, float* __restrict__ partials) {
+  __shared__ float s[7][kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  float vals[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const f3 vi = mul(add(xyz(v[i]), dtg), 1.f - kg);
+    v[i] = f4(vi, 0.f);
+    const float m = mass[i];
+    const f3 pm = mul(xyz(p[i]), m), vm = mul(vi, m);
+    vals[0] = m;
+    vals[1] = pm.x; vals[2] = pm.y; vals[3] = pm.z;
+    vals[4] = vm.x; vals[5] = vm.y; vals[6] = vm.z;
+  }
+  block_tree<7>(vals, s, partials + 7 * blockIdx.x);
+}
+
+// single thread: block sums in order -> xcm, vcm (state[0..5])
+__global__ void k_damp_mid(int nb, const float* __restrict__ partials,
+                           float* __restrict__ state) {
+  if (threadIdx.x != 0) return;
+  float t[7];
+  for (int k = 0; k < 7; ++k) t[k] = partials[k];
+  for (int b = 1; b < nb; ++b)
+    for (int k = 0; k < 7; ++k) t[k] = t[k] + partials[7 * b + k];
+  const f3 xcm = divs(mk3(t[1], t[2], t[3]), t[0]);
+  const f3 vcm = divs(mk3(t[4], t[5], t[6]), t[0]);
+  state[0] = xcm.x; state[1] = xcm.y; state[2] = xcm.z;
+  state[3] = vcm.x; state[4] = vcm.y; state[5] = vcm.z;
+}
+
+// partial sums of L = sum r x (v' m), i1 = sum (r.x^2, r.y^2, r.z^2) m,
+// i2 = -sum (r.x r.y, r.x r.z, r.y r.z) m
+__global__ void __launch_bounds__(kBlock) k_damp_sum2(int n, const float4* __restrict__ p,
+                                                      const float4* __restrict__ v,
+                                                      const float* __restrict__ mass,
+                                                      const float* __restrict__ state,
+                                                      float* __restrict__ partials) {
+  __shared__ float s[9][kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  float vals[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    const f3 xcm = mk3(state[0], state[1], state[2]);
+    const float m = mass[i];
+    const f3 r = sub(xyz(p[i]), xcm);
+    const f3 L = cross(r, mul(xyz(v[i]), m));
+    const f3 a = mul(mk3(r.x * r.x, r.y * r.y, r.z * r.z), m);
+    const f3 b = mul(neg(mk3(r.x * r.y, r.x * r.z, r.y * r.z)), m);
+    vals[0] = L.x; vals[1] = L.y; vals[2] = L.z;
+    vals[3] = a.x; vals[4] = a.y; vals[5] = a.z;
+    vals[6] = b.x; vals[7] = b.y; vals[8] = b.z;
+  }
+  block_tree<9>(vals, s, partials + 9 * blockIdx.x);
+}
+
+// single thread: omega = I^-1 L (state[6..8])
+__global__ void k_damp_final(int nb, const float* __restrict__ partials,
+                             float* __restrict__ state) {
+  if (threadIdx.x != 0) return;
+  float t[9];
+  for (int k = 0; k < 9; ++k) t[k] = partials[k];
+  for (int b = 1; b < nb; ++b)
+    for (int k = 0; k < 9; ++k) t[k] = t[k] + partials[9 * b + k];
+  const f3 L = mk3(t[0], t[1], t[2]), i1 = mk3(t[3], t[4], t[5]), i2 = mk3(t[6], t[7], t[8]);
+  const float a00 = i1.y + i1.z, a01 = i2.x, a02 = i2.y;
+  const float a11 = i1.x + i1.z, a12 = i2.z, a22 = i1.x + i1.y;
+  const float m00 = a00, m01 = a01, m02 = a02, m10 = a01, m11 = a11, m12 = a12, m20 = a02,
+              m21 = a12, m22 = a22;
+  const float inv_det = 1.f / (+m00 * (m11 * m22 - m21 * m12) - m10 * (m01 * m22 - m21 * m02) +
+                               m20 * (m01 * m12 - m11 * m02));
+  const float i00 = +(m11 * m22 - m21 * m12) * inv_det, i10 = -(m10 * m22 - m20 * m12) * inv_det,
+              i20 = +(m10 * m21 - m20 * m11) * inv_det, i01 = -(m01 * m22 - m21 * m02) * inv_det,
+              i11 = +(m00 * m22 - m20 * m02) * inv_det, i21 = -(m00 * m21 - m20 * m01) * inv_det,
+              i02 = +(m01 * m12 - m11 * m02) * inv_det, i12 = -(m00 * m12 - m10 * m02) * inv_det,
+              i22 = +(m00 * m11 - m10 * m01) * inv_det;
+  state[6] = i00 * L.x + i10 * L.y + i20 * L.z;
+  state[7] = i01 * L.x + i11 * L.y + i21 * L.z;
+  state[8] = i02 * L.x + i12 * L.y + i22 * L.z;
+}
+
+__global__ void __launch_bounds__(kBlock) k_damp_apply(int n, const float4* __restrict__ x,
+                                                       float4* __restrict__ p,
+                                                       float4* __restrict__ v,
+                                                       const float* __restrict__ state, float kd,
+                                                       float dt) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const f3 xcm = mk3(state[0], state[1], state[2]), vcm = mk3(state[3], state[4], state[5]),
+           om = mk3(state[6], state[7], state[8]);
+  const f3 r = sub(xyz(p[i]), xcm);
+  const f3 vi = xyz(v[i]);
+  const f3 dv = sub(add(vcm, cross(om, r)), vi);
+  const f3 vn = add(vi, mul(dv, kd));
+  v[i] = f4(vn, 0.f);
+  const float4 xi = x[i];
+  p[i] = f4(add(xyz(xi), mul(vn, dt)), xi.w);
+}
+
+// ---- solver set-up ------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_setup(int n, const float4* __restrict__ x,
+                                                  const float4* __restrict__ p, float gh,
+                                                  float th, int ground_on,
+                                                  float4* __restrict__ ground, int F,
+                                                  const int* __restrict__ fixed,
+                                                  const float* __restrict__ l0,
+                                                  const uint8_t* __restrict__ pinned,
+                                                  uint8_t* __restrict__ lflag) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const f3 pos = xyz(p[i]);
+  if (ground_on) {
+    float4 gq = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (pos.y <= gh + th) {
+      const f3 nc = mk3(0.f, 1.f, 0.f);
+      const f3 p0 = mk3(1.f, gh + th, 1.f);
+      const f3 l = sub(pos, xyz(x[i]));
+      const float d = dot(sub(p0, pos), nc) / dot(l, nc);
+      gq = f4(add(mul(l, d), pos), 1.f);
+    }
+    ground[i] = gq;
+  }
+  for (int j = 0; j < F; ++j) {
+    uint8_t f = 0;
+    if (!pinned[i]) {
+      const f3 q = xyz(p[fixed[j]]);
+      f = len(sub(pos, q)) > l0[(int64_t)i * F + j] ? 1 : 0;
+    }
+    lflag[(int64_t)i * F + j] = f;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sphere(int n, float4* __restrict__ x,
+                                                   float4* __restrict__ p, f3 c, float rad) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 xi = x[i], pi = p[i];
+  const f3 a = xyz(xi), b = xyz(pi);
+  f3 dir = sub(b, a);
+  if (is_zero(dir)) return;
+  const float l = len(dir);
+  dir = divs(dir, l);
+  const f3 m = sub(a, c);
+  const float bb = dot(m, dir);
+  const float cc = dot(m, m) - rad * rad;
+  if (cc > 0.f && bb > 0.f) return;
+  const float del = bb * bb - cc;
+  if (del < 0) return;
+  const float tt = -bb - sqrtf(del);
+  if (tt < 0.f) {
+    const f3 qc = add(a, mul(dir, tt));
+    const f3 q = add(c, mul(normalize(sub(qc, c)), rad));
+    const f3 nc = normalize(sub(q, c));
+    x[i] = f4(add(q, mul(nc, 0.0001f)), xi.w);
+    p[i] = f4(q, pi.w);
+    return;
+  }
+  if (tt > l) return;
+  const f3 qc = add(a, mul(dir, tt));
+  const f3 nc = normalize(sub(qc, c));
+  p[i] = f4(add(qc, mul(nc, 0.0001f)), pi.w);
+}
+
+// ---- constraint corrections for one vertex role ---------------------------------
+__device__ __forceinline__ f3 stretch_corr(int2 ids, float2 l0kp, int role,
+                                           const float4* __restrict__ p) {
+  const float4 a = p[ids.x], b = p[ids.y];
+  const float w1 = a.w, w2 = b.w, ws = w1 + w2;
+  if (ws == 0) return mk3(0, 0, 0);
+  const f3 diff = sub(xyz(a), xyz(b));
+  const float l = len(diff);
+  if (l <= 1e-8f) return mk3(0, 0, 0);
+  const f3 nn = divs(diff, l);
+  const f3 tt = divs(mul(mul(nn, l - l0kp.x), l0kp.y), ws);
+  return role == 0 ? mul(tt, -w1) : mul(tt, w2);
+}
+
+// vector arithmetic in T (float or double) with the operation order of cm_math.hpp
+template <class T>
+struct tv {
+  T x, y, z;
+};
+template <class T> __device__ __forceinline__ tv<T> tadd(tv<T> a, tv<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> __device__ __forceinline__ tv<T> tsub(tv<T> a, tv<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> __device__ __forceinline__ tv<T> tneg(tv<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> __device__ __forceinline__ tv<T> tmul(tv<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> __device__ __forceinline__ tv<T> tdiv(tv<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> __device__ __forceinline__ T tdot(tv<T> a, tv<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> __device__ __forceinline__ tv<T> tcross(tv<T> a, tv<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> __device__ __forceinline__ T tsqrt(T v);
+template <> __device__ __forceinline__ float tsqrt<float>(float v) { return sqrtf(v); }
+template <> __device__ __forceinline__ double tsqrt<double>(double v) { return sqrt(v); }
+template <class T> __device__ __forceinline__ T tacos(T v);
+template <> __device__ __forceinline__ float tacos<float>(float v) { return acosf(v); }
+template <> __device__ __forceinline__ double tacos<double>(double v) { return acos(v); }
+template <class T> __device__ __forceinline__ T tlen(tv<T> a) { return tsqrt<T>(a.x * a.x + a.y * a.y + a.z * a.z); }
+template <class T> __device__ __forceinline__ tv<T> tfrom(float4 a) { return {(T)a.x, (T)a.y, (T)a.z}; }
+template <class T> __device__ __forceinline__ f3 tto(tv<T> a) { return mk3((float)a.x, (float)a.y, (float)a.z); }
+
+// Physics/CudaPBD.cu:1024-1147 for one vertex role, evaluated in T
+template <class T>
+__device__ f3 bend_corr(int4 ids, float2 kpphi0, int role, int abs_mode,
+                        const float4* __restrict__ p) {
+  const float4 a1 = p[ids.x], a2 = p[ids.y], a3 = p[ids.z], a4 = p[ids.w];
+  const T w1 = (T)a1.w, w2 = (T)a2.w, w3 = (T)a3.w, w4 = (T)a4.w;
+  tv<T> p2 = tfrom<T>(a2), p3 = tfrom<T>(a3), p4 = tfrom<T>(a4);
+  if (!abs_mode) {
+    const tv<T> p1 = tfrom<T>(a1);
+    p2 = tsub(p2, p1);
+    p3 = tsub(p3, p1);
+    p4 = tsub(p4, p1);
+  }
+  const T kp = (T)kpphi0.x, phi0 = (T)kpphi0.y;
+  const tv<T> c23 = tcross(p2, p3), c24 = tcross(p2, p4);
+  const T l23 = tlen(c23), l24 = tlen(c24);
+  const T eps = (T)1e-8f;
+  if (l23 <= eps || l24 <= eps) return mk3(0, 0, 0);
+  const tv<T> n1 = tdiv(c23, l23), n2 = tdiv(c24, l24);
+  const T d = fmin(fmax((T)-1, tdot(n1, n2)), (T)1);
+  T phi = tacos<T>(d);
+  const T PIf = (T)3.14159265359f;
+  if (d == (T)-1) {
+    phi = PIf;
+    if (phi == phi0) return mk3(0, 0, 0);
+    if (role == 0) return tto(tdiv(n1, (T)100));
+    if (role == 1) return tto(tdiv(n2, (T)100));
+    return mk3(0, 0, 0);
+  }
+  if (d == (T)1) {
+    phi = (T)0;
+    if (phi == phi0) return mk3(0, 0, 0);
+  }
+  const tv<T> q3 = tdiv(tadd(tcross(p2, n2), tmul(tcross(n1, p2), d)), l23);
+  const tv<T> q4 = tdiv(tadd(tcross(p2, n1), tmul(tcross(n2, p2), d)), l24);
+  const tv<T> q2 = tsub(tneg(tdiv(tadd(tcross(p3, n2), tmul(tcross(n1, p3), d)), l23)),
+                        tdiv(tadd(tcross(p4, n1), tmul(tcross(n2, p4), d)), l24));
+  const tv<T> q1 = tsub(tsub(tneg(q2), q3), q4);
+  const T term = tsqrt<T>((T)1 - d * d) * (phi - phi0) * kp;
+  const T s1 = tdot(q1, q1), s2 = tdot(q2, q2), s3 = tdot(q3, q3), s4 = tdot(q4, q4);
+  const T sf = w1 * s1 + w2 * s2 + w3 * s3 + w4 * s4;
+  if (sf <= eps) return mk3(0, 0, 0);
+  const T ts = term / sf;
+  switch (role) {
+    case 0: return tto(tmul(q1, -(w1 * ts)));
+    case 1: return tto(tmul(q2, -(w2 * ts)));
+    case 2: return tto(tmul(q3, -(w3 * ts)));
+    default: return tto(tmul(q4, -(w4 * ts)));
+  }
+}
+
+__device__ __forceinline__ f3 vf_corr(int4 v, float4 w, int role, float th,
+                                      const float4* __restrict__ p) {
+  const float b0 = fclamp(-w.y, 0.f, 1.f), b1 = fclamp(-w.z, 0.f, 1.f),
+              b2 = fclamp(-w.w, 0.f, 1.f);
+  const float4 A = p[v.x], B0 = p[v.y], B1 = p[v.z], B2 = p[v.w];
+  const float ww = A.w, w0 = B0.w, w1 = B1.w, w2 = B2.w;
+  const f3 q = add(add(mul(xyz(B0), b0), mul(xyz(B1), b1)), mul(xyz(B2), b2));
+  f3 nn = sub(xyz(A), q);
+  const float l = len(nn);
+  nn = divs(nn, l);
+  const float cc = l - th;
+  float s = ww + w0 * b0 * b0 + w1 * b1 * b1 + w2 * b2 * b2;
+  if (s == 0) return mk3(0, 0, 0);
+  s = cc / s;
+  s = cc < 0 ? s : 0.f;
+  switch (role) {
+    case 0: return mul(nn, -s * ww);
+    case 1: return mul(mul(neg(nn), b0), -s * w0);
+    case 2: return mul(mul(neg(nn), b1), -s * w1);
+    default: return mul(mul(neg(nn), b2), -s * w2);
+  }
+}
+
+__device__ __forceinline__ f3 ee_corr(int4 v, float4 w, int role, float th,
+                                      const float4* __restrict__ p) {
+  const float b0 = w.x, b1 = w.y, b2 = -w.z, b3 = -w.w;
+  const float4 P0 = p[v.x], P1 = p[v.y], P2 = p[v.z], P3 = p[v.w];
+  const float w0 = P0.w, w1 = P1.w, w2 = P2.w, w3 = P3.w;
+  const f3 q0 = add(mul(xyz(P0), b0), mul(xyz(P1), b1));
+  const f3 q1 = add(mul(xyz(P2), b2), mul(xyz(P3), b3));
+  f3 nn = sub(q0, q1);
+  const float l = len(nn);
+  nn = divs(nn, l);
+  const float cc = l - th;
+  float s = w0 * b0 * b0 + w1 * b1 * b1 + w2 * b2 * b2 + w3 * b3 * b3;
+  if (s == 0) return mk3(0, 0, 0);
+  s = cc / s;
+  s = cc < 0 ? s : 0.f;
+  switch (role) {
+    case 0: return mul(mul(nn, b0), -s * w0);
+    case 1: return mul(mul(nn, b1), -s * w1);
+    case 2: return mul(mul(neg(nn), b2), -s * w2);
+    default: return mul(mul(neg(nn), b3), -s * w3);
+  }
+}
+
+struct SolveArgs {
+  int n, F, abs_mode, ground_on, bend_fp64;
+  float th;
+  const float4* ground;
+  const int* lra_fixed;
+  const float* lra_l0;
+  const uint8_t* lra_flag;
+  const int* vprefix;
+  const uint32_t* vslot;
+  const int2* sid;
+  const float2* s_l0kp;
+  const int4* bid;
+  const float2* b_kpphi0;
+  const int* dprefix;
+  const uint32_t* drec;
+  const int* d_nvf;
+  const int4 *vf_v, *ee_v;
+  const float4 *vf_w, *ee_w;
+};
+
+__global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __restrict__ pin,
+                                                  float4* __restrict__ pout) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const float4 pi = pin[i];
+  f3 total = mk3(0, 0, 0);
+  int cnt = 0;
+  if (a.ground_on) {
+    const float4 gq = a.ground[i];
+    if (gq.w != 0.f) {
+      const f3 nc = mk3(0.f, 1.f, 0.f);
+      const float cv = dot(sub(xyz(pi), xyz(gq)), nc);
+      if (cv < 0) {
+        total = add(total, mul(nc, -cv));
+        cnt += 1;
+      }
+    }
+  }
+  if (a.F) {
+    f3 acc = mk3(0, 0, 0);
+    int c = 0;
+    for (int j = 0; j < a.F; ++j) {
+      if (a.lra_flag[(int64_t)i * a.F + j]) {
+        const f3 diff = sub(xyz(pi), xyz(pin[a.lra_fixed[j]]));
+        const float l = len(diff);
+        const f3 nn = divs(diff, l);
+        acc = add(acc, mul(nn, -(l - a.lra_l0[(int64_t)i * a.F + j])));
+        ++c;
+      }
+    }
+    total = add(total, acc);
+    cnt += c;
+  }
+  {
+    const int b = a.vprefix[i], e = a.vprefix[i + 1];
+    f3 acc = mk3(0, 0, 0);
+    for (int k = b; k < e; ++k) {
+      const uint32_t s = a.vslot[k];
+      const int idx = (int)(s & 0x1fffffffu);
+      const int role = (int)((s >> 29) & 3u);
+      f3 c;
+      if (s & 0x80000000u)
+        c = a.bend_fp64 ? bend_corr<double>(a.bid[idx], a.b_kpphi0[idx], role, a.abs_mode, pin)
+                        : bend_corr<float>(a.bid[idx], a.b_kpphi0[idx], role, a.abs_mode, pin);
+      else
+        c = stretch_corr(a.sid[idx], a.s_l0kp[idx], role, pin);
+      acc = add(acc, c);
+    }
+    total = add(total, acc);
+    cnt += e - b;
+  }
+  if (a.dprefix) {
+    const int b = a.dprefix[i], e = a.dprefix[i + 1];
+    if (e > b) {
+      const int nvf = *a.d_nvf;
+      f3 acc = mk3(0, 0, 0);
+      int c = 0;
+      for (int k = b; k < e; ++k) {
+        const uint32_t r = a.drec[k];
+        const int ci = (int)(r >> 2), role = (int)(r & 3u);
+        const f3 val = ci < nvf ? vf_corr(a.vf_v[ci], a.vf_w[ci], role, a.th, pin)
+                                : ee_corr(a.ee_v[ci - nvf], a.ee_w[ci - nvf], role, a.th, pin);
+        if (!is_zero(val)) {
+          acc = add(acc, val);
+          ++c;
+        }
+      }
+      total = add(total, acc);
+      cnt += c;
+    }
+  }
+  pout[i] = f4(add(xyz(pi), divs(total, (float)cnt)), pi.w);
+}
+
+__global__ void __launch_bounds__(kBlock) k_integrate(int n, const float4* __restrict__ pfin,
+                                                      float4* __restrict__ x,
+                                                      float4* __restrict__ v,
+                                                      float4* __restrict__ p,
+                                                      const uint8_t* __restrict__ pinned,
+                                                      float dt, int clamp, f3 lo, f3 hi) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float4 pi = pfin[i];
+  if (pinned[i]) {
+    v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  } else {
+    const float4 xi = x[i];
+    v[i] = f4(divs(sub(xyz(pi), xyz(xi)), dt), 0.f);
+    x[i] = make_float4(pi.x, pi.y, pi.z, xi.w);
+  }
+  if (clamp) {
+    pi.x = fminf(fmaxf(pi.x, lo.x), hi.x);
+    pi.y = fminf(fmaxf(pi.y, lo.y), hi.y);
+    pi.z = fminf(fmaxf(pi.z, lo.z), hi.z);
+  }
+  p[i] = pi;
+}
+
+}  // namespace
+
+int launch_predict(const PbdBufs& b, const PbdParams& pp, hipStream_t s) {
+  const int nb = nblocks(b.n);
+  if (pp.flags & CM_F_DAMPING) {
+    hipLaunchKernelGGL(k_damp_sum1, dim3(nb), dim3(kBlock), 0, s, b.n, b.p, b.v, b.mass,
+                       mul(pp.g, pp.dt), pp.kg, b.damp_partials);
+    hipLaunchKernelGGL(k_damp_mid, dim3(1), dim3(64), 0, s, nb, b.damp_partials, b.damp_state);
+    hipLaunchKernelGGL(k_damp_sum2, dim3(nb), dim3(kBlock), 0, s, b.n, b.p, b.v, b.mass,
+                       b.damp_state, b.damp_partials);
+    hipLaunchKernelGGL(k_damp_final, dim3(1), dim3(64), 0, s, nb, b.damp_partials, b.damp_state);
+    hipLaunchKernelGGL(k_damp_apply, dim3(nb), dim3(kBlock), 0, s, b.n, b.x, b.p, b.v,
+                       b.damp_state, pp.kd, pp.dt);
+  } else {
+    hipLaunchKernelGGL(k_predict, dim3(nb), dim3(kBlock), 0, s, b.n, b.x, b.v, b.p, pp.dt, pp.g);
+  }
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+static SolveArgs solve_args(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb) {
+  const int ground_on = (pp.flags & CM_F_GROUND) ? 1 : 0;
+  const int F = (pp.flags & CM_F_LRA) ? b.F : 0;
+  SolveArgs a;
+  a.n = b.n;
+  a.F = F;
+  a.abs_mode = (pp.ref_compat & CM_RC_BEND_ABS) ? 1 : 0;
+  a.bend_fp64 = (pp.flags & CM_F_BEND_FP64) ? 1 : 0;
+  a.ground_on = ground_on;
+  a.th = pp.th;
+  a.ground = b.ground;
+  a.lra_fixed = b.lra_fixed;
+  a.lra_l0 = b.lra_l0;
+  a.lra_flag = b.lra_flag;
+  a.vprefix = b.vprefix;
+  a.vslot = b.vslot;
+  a.sid = b.sid;
+  a.s_l0kp = b.s_l0kp;
+  a.bid = b.bid;
+  a.b_kpphi0 = b.b_kpphi0;
+  const bool coll = (pp.flags & CM_F_SELF_COLLISION) && nb.dprefix;
+  a.dprefix = coll ? nb.dprefix : nullptr;
+  a.drec = nb.drec;
+  a.d_nvf = nb.d_nvf;
+  a.vf_v = nb.vf_v;
+  a.ee_v = nb.ee_v;
+  a.vf_w = nb.vf_w;
+  a.ee_w = nb.ee_w;
+  return a;
+}
+
+// `iters` Jacobi iterations ping-ponging p -> p2 -> p ...
+static int launch_iterations(const PbdBufs& b, const SolveArgs& a, int iters, hipStream_t s) {
+  const int g = nblocks(b.n);
+  float4* src = b.p;
+  float4* dst = b.p2;
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_solve, dim3(g), dim3(kBlock), 0, s, a, src, dst);
+    std::swap(src, dst);
+  }
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_solver(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, hipStream_t s) {
+  const int g = nblocks(b.n);
+  const int ground_on = (pp.flags & CM_F_GROUND) ? 1 : 0;
+  const int F = (pp.flags & CM_F_LRA) ? b.F : 0;
+  hipLaunchKernelGGL(k_setup, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.gh, pp.th, ground_on,
+                     b.ground, F, b.lra_fixed, b.lra_l0, b.pinned, b.lra_flag);
+  if (pp.flags & CM_F_SPHERE)
+    hipLaunchKernelGGL(k_sphere, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.sc, pp.sr + pp.th);
+  CM_LAUNCH_CHECK();
+  return launch_iterations(b, solve_args(b, pp, nb), pp.iters, s);
+}
+
+int launch_solve_only(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, int iters,
+                      hipStream_t s) {
+  return launch_iterations(b, solve_args(b, pp, nb), iters, s);
+}
+
+int launch_integrate(const PbdBufs& b, const PbdParams& pp, hipStream_t s) {
+  const float4* pfin = (pp.iters & 1) ? b.p2 : b.p;
+  hipLaunchKernelGGL(k_integrate, dim3(nblocks(b.n)), dim3(kBlock), 0, s, b.n, pfin, b.x, b.v,
+                     b.p, b.pinned, pp.dt, (pp.flags & CM_F_WORLD_CLAMP) ? 1 : 0, pp.wmin,
+                     pp.wmax);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+}  // namespace cm

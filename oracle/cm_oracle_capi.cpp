@@ -1,0 +1,298 @@
+// ORACLE — C entry points over cm_oracle.hpp for the Python test harness
+// (tests/ and the bench's cpu_baseline leg load this through ctypes).
+// Test infrastructure only: the product never links or calls it.
+#include <chrono>
+#include <memory>
+
+#include "cm_oracle.hpp"
+
+using namespace orc;
+
+namespace {
+struct Handle {
+  int precision = 32;
+  std::unique_ptr<Sim<float>> f;
+  std::unique_ptr<Sim<double>> d;
+};
+
+template <class R, class Fn>
+auto with(Handle* h, Fn fn) {
+  return h->precision == 64 ? fn(*h->d) : fn(*h->f);
+}
+
+template <class T>
+int64_t copy_out(const std::vector<T>& src, T* out, int64_t cap) {
+  int64_t n = (int64_t)src.size();
+  if (out) std::memcpy(out, src.data(), sizeof(T) * (size_t)std::min(n, cap));
+  return n;
+}
+}  // namespace
+
+extern "C" {
+
+void* orc_create(const cm_params* prm, const float* xyz, int n, const uint32_t* tri, int t,
+                 const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
+                 int n_dist, const float* k_dist, const float* k_bend, int precision) {
+  Handle* h = new Handle();
+  h->precision = precision == 64 ? 64 : 32;
+  if (h->precision == 64) {
+    h->d.reset(new Sim<double>());
+    h->d->create(*prm, xyz, n, tri, t, pinned, inv_mass, dist_pairs, n_dist, k_dist, k_bend);
+  } else {
+    h->f.reset(new Sim<float>());
+    h->f->create(*prm, xyz, n, tri, t, pinned, inv_mass, dist_pairs, n_dist, k_dist, k_bend);
+  }
+  return h;
+}
+
+void orc_destroy(void* hp) { delete (Handle*)hp; }
+
+int orc_step(void* hp, int substeps) {
+  Handle* h = (Handle*)hp;
+  for (int s = 0; s < substeps; ++s) with<float>(h, [](auto& sim) { sim.step(); return 0; });
+  return 0;
+}
+
+// Times `substeps` substeps; returns seconds (CPU baseline leg).
+double orc_time_steps(void* hp, int substeps) {
+  auto t0 = std::chrono::steady_clock::now();
+  orc_step(hp, substeps);
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+int orc_set_sphere(void* hp, const float* c, float r) {
+  Handle* h = (Handle*)hp;
+  with<float>(h, [&](auto& sim) {
+    sim.sphere_on = true;
+    sim.sphere_c = mk(c[0], c[1], c[2]);
+    sim.sphere_r = r;
+    return 0;
+  });
+  return 0;
+}
+
+// x, p, v: 3N doubles each (p may be NULL: p = x); tri: 4T ints or NULL.
+int orc_set_state(void* hp, const double* x, const double* p, const double* v, const int32_t* tri) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) {
+    typedef typename std::remove_reference<decltype(sim.x[0].x)>::type R;
+    for (int i = 0; i < sim.n; ++i) {
+      sim.x[i] = mk<R>(R(x[3 * i]), R(x[3 * i + 1]), R(x[3 * i + 2]));
+      const double* pp = p ? p : x;
+      sim.p[i] = mk<R>(R(pp[3 * i]), R(pp[3 * i + 1]), R(pp[3 * i + 2]));
+      sim.v[i] = mk<R>(R(v[3 * i]), R(v[3 * i + 1]), R(v[3 * i + 2]));
+    }
+    if (tri)
+      for (int i = 0; i < sim.t; ++i) sim.tri[i] = Tri{tri[4 * i], tri[4 * i + 1], tri[4 * i + 2], tri[4 * i + 3]};
+    return 0;
+  });
+}
+
+// parents carry over between substeps (entries no node claims keep their value)
+int orc_set_parents(void* hp, const int32_t* parents) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) {
+    std::copy(parents, parents + sim.parents.size(), sim.parents.begin());
+    return 0;
+  });
+}
+
+int orc_get_state(void* hp, double* x, double* p, double* v) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) {
+    for (int i = 0; i < sim.n; ++i) {
+      if (x) { x[3 * i] = sim.x[i].x; x[3 * i + 1] = sim.x[i].y; x[3 * i + 2] = sim.x[i].z; }
+      if (p) { p[3 * i] = sim.p[i].x; p[3 * i + 1] = sim.p[i].y; p[3 * i + 2] = sim.p[i].z; }
+      if (v) { v[3 * i] = sim.v[i].x; v[3 * i + 1] = sim.v[i].y; v[3 * i + 2] = sim.v[i].z; }
+    }
+    return 0;
+  });
+}
+
+// counts: n, t, S, B, P, C_vf, C_ee, slots
+void orc_counts(void* hp, int64_t* out) {
+  Handle* h = (Handle*)hp;
+  with<float>(h, [&](auto& sim) {
+    out[0] = sim.n;
+    out[1] = sim.t;
+    out[2] = (int64_t)sim.tp.sid1.size();
+    out[3] = (int64_t)sim.tp.bkp.size();
+    out[4] = (int64_t)sim.pairs.size() / 2;
+    out[5] = (int64_t)sim.vf.size();
+    out[6] = (int64_t)sim.ee.size();
+    out[7] = (int64_t)sim.tp.prefix.back();
+    return 0;
+  });
+}
+
+uint32_t orc_warnings(void* hp) {
+  return with<float>((Handle*)hp, [](auto& sim) { return sim.warn; });
+}
+
+// which: 0 tri(4T) 1 lefts 2 rights 3 parents 4 rmll 5 rmlr 6 pairs(2P)
+//        10 sid(2S) 11 bid(4B) 12 prefix(N+1) 13 soff(2S) 14 boff(4B) 15 init tri(4T)
+int64_t orc_get_i32(void* hp, int which, int32_t* out, int64_t cap) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) -> int64_t {
+    std::vector<int32_t> tmp;
+    switch (which) {
+      case 0:
+      case 15: {
+        const auto& tr = which == 0 ? sim.tri : sim.tp.tri;
+        for (const Tri& q : tr) {
+          tmp.push_back(q.v1);
+          tmp.push_back(q.v2);
+          tmp.push_back(q.v3);
+          tmp.push_back(q.feat);
+        }
+        return copy_out(tmp, out, cap);
+      }
+      case 1: return copy_out(sim.lefts, out, cap);
+      case 2: return copy_out(sim.rights, out, cap);
+      case 3: return copy_out(sim.parents, out, cap);
+      case 4: return copy_out(sim.rmll, out, cap);
+      case 5: return copy_out(sim.rmlr, out, cap);
+      case 6: return copy_out(sim.pairs, out, cap);
+      case 10:
+        for (size_t i = 0; i < sim.tp.sid1.size(); ++i) {
+          tmp.push_back(sim.tp.sid1[i]);
+          tmp.push_back(sim.tp.sid2[i]);
+        }
+        return copy_out(tmp, out, cap);
+      case 11:
+        for (size_t i = 0; i < sim.tp.bkp.size(); ++i)
+          for (int r = 0; r < 4; ++r) tmp.push_back(sim.tp.bid[r][i]);
+        return copy_out(tmp, out, cap);
+      case 12: return copy_out(sim.tp.prefix, out, cap);
+      case 13:
+        for (size_t i = 0; i < sim.tp.sid1.size(); ++i) {
+          tmp.push_back(sim.tp.soff[0][i]);
+          tmp.push_back(sim.tp.soff[1][i]);
+        }
+        return copy_out(tmp, out, cap);
+      case 14:
+        for (size_t i = 0; i < sim.tp.bkp.size(); ++i)
+          for (int r = 0; r < 4; ++r) tmp.push_back(sim.tp.boff[r][i]);
+        return copy_out(tmp, out, cap);
+      default: return -1;
+    }
+  });
+}
+
+// which: 0 sl0 1 skp 2 bkp 3 bphi0 4 lra_l0
+int64_t orc_get_f32(void* hp, int which, float* out, int64_t cap) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) -> int64_t {
+    switch (which) {
+      case 0: return copy_out(sim.tp.sl0, out, cap);
+      case 1: return copy_out(sim.tp.skp, out, cap);
+      case 2: return copy_out(sim.tp.bkp, out, cap);
+      case 3: return copy_out(sim.tp.bphi0, out, cap);
+      case 4: return copy_out(sim.tp.lra_l0, out, cap);
+      default: return -1;
+    }
+  });
+}
+
+int64_t orc_get_keys(void* hp, uint64_t* out, int64_t cap) {
+  return with<float>((Handle*)hp, [&](auto& sim) { return copy_out(sim.keys, out, cap); });
+}
+int64_t orc_get_perm(void* hp, uint32_t* out, int64_t cap) {
+  return with<float>((Handle*)hp, [&](auto& sim) { return copy_out(sim.perm, out, cap); });
+}
+// 3 floats per node, 2T-1 nodes
+int64_t orc_get_aabb(void* hp, float* mins, float* maxs, int64_t cap) {
+  return with<float>((Handle*)hp, [&](auto& sim) -> int64_t {
+    int64_t n = (int64_t)sim.amin.size();
+    for (int64_t i = 0; i < std::min(n, cap); ++i) {
+      mins[3 * i] = sim.amin[i].x; mins[3 * i + 1] = sim.amin[i].y; mins[3 * i + 2] = sim.amin[i].z;
+      maxs[3 * i] = sim.amax[i].x; maxs[3 * i + 1] = sim.amax[i].y; maxs[3 * i + 2] = sim.amax[i].z;
+    }
+    return n;
+  });
+}
+int64_t orc_get_contacts(void* hp, int ee, cm_contact* out, int64_t cap) {
+  return with<float>((Handle*)hp, [&](auto& sim) { return copy_out(ee ? sim.ee : sim.vf, out, cap); });
+}
+
+// ---- stage functions on plain arrays ------------------------------------
+void orc_morton63(int n, const float* c3, uint64_t* out) {
+  for (int i = 0; i < n; ++i) out[i] = morton63(mk(c3[3 * i], c3[3 * i + 1], c3[3 * i + 2]));
+}
+void orc_morton30(int n, const float* c3, const float* lo, const float* hi, uint64_t* out) {
+  F3 l = mk(lo[0], lo[1], lo[2]), hh = mk(hi[0], hi[1], hi[2]);
+  for (int i = 0; i < n; ++i) out[i] = morton30(mk(c3[3 * i], c3[3 * i + 1], c3[3 * i + 2]), l, hh);
+}
+// keys/vals sorted stably by key, in place
+void orc_stable_sort(int n, uint64_t* keys, uint32_t* vals) {
+  std::vector<uint32_t> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+  std::vector<uint64_t> k(n);
+  std::vector<uint32_t> v(n);
+  for (int i = 0; i < n; ++i) {
+    k[i] = keys[idx[i]];
+    v[i] = vals[idx[i]];
+  }
+  std::memcpy(keys, k.data(), 8 * (size_t)n);
+  std::memcpy(vals, v.data(), 4 * (size_t)n);
+}
+void orc_build(int t, const uint64_t* keys, int midpoint, int32_t* lefts, int32_t* rights,
+               int32_t* parents, uint32_t* warn) {
+  lbvh_build(t, keys, midpoint != 0, lefts, rights, parents, warn);
+}
+static void to_f3(const float* a, int64_t n, std::vector<F3>& out) {
+  out.resize(n);
+  for (int64_t i = 0; i < n; ++i) out[i] = mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+}
+// amin3/amax3: (2T-1)*3 floats with leaves filled; internal nodes written.
+void orc_refit(int t, const int32_t* lefts, const int32_t* rights, float* amin3, float* amax3,
+               int32_t* rmll, int32_t* rmlr) {
+  std::vector<F3> mn, mx;
+  to_f3(amin3, 2 * (int64_t)t - 1, mn);
+  to_f3(amax3, 2 * (int64_t)t - 1, mx);
+  lbvh_refit(t, lefts, rights, mn.data(), mx.data(), rmll, rmlr);
+  for (int64_t i = 0; i < 2 * (int64_t)t - 1; ++i) {
+    amin3[3 * i] = mn[i].x; amin3[3 * i + 1] = mn[i].y; amin3[3 * i + 2] = mn[i].z;
+    amax3[3 * i] = mx[i].x; amax3[3 * i + 1] = mx[i].y; amax3[3 * i + 2] = mx[i].z;
+  }
+}
+// Returns the number of pairs written to out (2 ints each, capacity 2*T*cap).
+int64_t orc_pairs(int t, const int32_t* lefts, const int32_t* rights, const float* amin3,
+                  const float* amax3, const int32_t* rmll, const int32_t* rmlr, int cap,
+                  int32_t* out, uint32_t* warn) {
+  std::vector<F3> mn, mx;
+  to_f3(amin3, 2 * (int64_t)t - 1, mn);
+  to_f3(amax3, 2 * (int64_t)t - 1, mx);
+  int64_t total = 0;
+  std::vector<int32_t> buf(2 * (size_t)cap);
+  for (int i = 0; i < t; ++i) {
+    bool capped = false;
+    int c = lbvh_leaf_pairs(t, i, lefts, rights, mn.data(), mx.data(), rmll, rmlr, cap, buf.data(),
+                            warn, &capped);
+    if (capped && warn) *warn |= CM_WARN_PAIR_CAP;
+    std::memcpy(out + 2 * total, buf.data(), sizeof(int32_t) * 2 * (size_t)c);
+    total += c;
+  }
+  return total;
+}
+int orc_coplanarity_times(const float* x12, const float* v12, float* times) {
+  F3 x[4], v[4];
+  for (int k = 0; k < 4; ++k) {
+    x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
+    v[k] = mk(v12[3 * k], v12[3 * k + 1], v12[3 * k + 2]);
+  }
+  return coplanarity_times(x[0], x[1], x[2], x[3], v[0], v[1], v[2], v[3], times);
+}
+int orc_test_vf(const float* x12, float th, cm_contact* c) {
+  F3 x[4];
+  for (int k = 0; k < 4; ++k) x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
+  return test_vf(x[0], x[1], x[2], x[3], *c, th);
+}
+int orc_test_ee(const float* x12, float th, cm_contact* c) {
+  F3 x[4];
+  for (int k = 0; k < 4; ++k) x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
+  return test_ee(x[0], x[1], x[2], x[3], *c, th, 0.000000001f);
+}
+}

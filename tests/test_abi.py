@@ -1,0 +1,58 @@
+"""The C-ABI library loads without a GPU and exports every function that
+include/clothmi.h declares; the ctypes mirror matches the header's layout."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from licenta_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "clothmi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_abi.LIB_PATH):
+        from licenta_amd import build
+        build.build()
+    lib = C.CDLL(_abi.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_abi.EXPORTS)
+
+
+def test_struct_layouts():
+    assert C.sizeof(_abi.CmParams) == 144
+    assert C.sizeof(_abi.CmContact) == 48
+    p = _abi.make_params(__import__("licenta_amd.configs", fromlist=["x"]).default_params())
+    assert abs(p.gravity[1] + 0.0981) < 1e-7 and p.morton_bits == 63 and p.pair_cap == 64
+
+
+def test_default_params_match_library():
+    lib = _abi.lib()
+    p = _abi.CmParams()
+    lib.cm_default_params(C.byref(p))
+    q = _abi.make_params(__import__("licenta_amd.configs", fromlist=["x"]).default_params())
+    assert bytes(p) == bytes(q)
+
+
+def test_create_fails_cleanly_on_bad_input():
+    lib = _abi.lib()
+    p = _abi.CmParams()
+    lib.cm_default_params(C.byref(p))
+    h = C.c_void_p()
+    xyz = np.zeros((3, 3), np.float32)
+    tri = np.array([[0, 1, 5]], np.uint32)  # index out of range
+    inv = np.ones(3, np.float32)
+    rc = lib.cm_create(C.byref(p), _abi.ptr(xyz), 3, _abi.ptr(tri), 1, None, _abi.ptr(inv), None, 0,
+                       None, None, C.byref(h))
+    assert rc == -1 and h.value is None
+    assert b"out of range" in lib.cm_last_error()
