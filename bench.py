@@ -83,6 +83,13 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     counts = body.counts()
     warn = body.warnings()
+    # per-stage split of the last substeps (eager launches between HIP events)
+    stage_names = ("predict", "triangles", "sort", "tree", "pairs", "narrow", "solver", "integrate")
+    stage = (C.c_float * 8)()
+    nstage = 10
+    if lib().cm_stage_times(body.h, nstage, stage):
+        raise RuntimeError("cm_stage_times failed")
+    stage_ms = {k: round(stage[i] / nstage, 4) for i, k in enumerate(stage_names)}
     # dominant kernel, timed live with HIP events on the body's stream
     avg = C.c_float()
     rc = lib().cm_time_kernel(body.h, 0, args.kernel_reps, C.byref(avg))
@@ -123,6 +130,7 @@ def main():
         substep_roofline_frac=per_substep / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         last_substep=dict(pairs=counts["pairs"], vf=counts["vf"], ee=counts["ee"],
                           warnings=warn),
+        stage_ms_after_run=stage_ms,
     )
     body.close()
     if not args.no_cpu:

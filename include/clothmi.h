@@ -193,6 +193,21 @@ int cm_query_contacts(cm_body* body, cm_contact* vf, int32_t cap_vf, int32_t* n_
 #define CM_KERNEL_SUBSTEP 1
 int cm_time_kernel(cm_body* body, int32_t which, int32_t reps, float* avg_ms);
 
+/* Per-stage timing of `substeps` substeps run eagerly on the body's stream with a
+ * HIP event between stages, as the reference's GPUBenchmark stage timers
+ * (Simulation/GPUBenchmark.cu:14-77, stage names at Physics/CudaDeformableBody.cpp:556-594).
+ * ms[CM_NUM_STAGES] receives the summed milliseconds per stage. */
+#define CM_STAGE_PREDICT 0    /* external forces, damping, Euler prediction   */
+#define CM_STAGE_TRIANGLES 1  /* swept AABBs and Morton keys                  */
+#define CM_STAGE_SORT 2       /* radix sort and triangle gather               */
+#define CM_STAGE_TREE 3       /* Karras build and refit                       */
+#define CM_STAGE_PAIRS 4      /* traversal and pair compaction                */
+#define CM_STAGE_NARROW 5     /* candidates, CCD, contact compaction, records */
+#define CM_STAGE_SOLVER 6     /* constraint set-up and Jacobi iterations      */
+#define CM_STAGE_INTEGRATE 7  /* velocity update and world clamp              */
+#define CM_NUM_STAGES 8
+int cm_stage_times(cm_body* body, int32_t substeps, float* ms);
+
 /* ---- unit-level LBVH entry points on caller device buffers -------------- */
 /* Swept leaf AABBs (written to leaf slots [T-1, 2T-1) of d_aabb_min/max) and
  * Morton keys for triangles d_tri in their current order. */

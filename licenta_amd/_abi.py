@@ -114,6 +114,7 @@ def lib():
     L.cm_lbvh_pairs_temp_bytes.restype = C.c_size_t
     L.cm_lbvh_pairs.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, C.c_size_t, vp]
     L.cm_time_kernel.argtypes = [vp, i32, i32, C.POINTER(C.c_float)]
+    L.cm_stage_times.argtypes = [vp, i32, vp]
     _lib = L
     return L
 
@@ -131,5 +132,5 @@ EXPORTS = (
     "cm_set_state", "cm_get_counts", "cm_get_buffer", "cm_get_warnings", "cm_query_pairs",
     "cm_query_contacts", "cm_tri_update", "cm_lbvh_sort_temp_bytes", "cm_lbvh_sort",
     "cm_lbvh_build_temp_bytes", "cm_lbvh_build", "cm_lbvh_refit", "cm_lbvh_pairs_temp_bytes",
-    "cm_lbvh_pairs", "cm_time_kernel",
+    "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times",
 )

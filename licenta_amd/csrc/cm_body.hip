@@ -316,32 +316,50 @@ PbdParams pbd_params(const cm_body* b) {
 }
 
 // One substep on b->stream (Physics/CudaDeformableBody.cpp:550-627 stage order).
-int record_substep(cm_body* b) {
+// ev (optional): CM_NUM_STAGES + 1 events recorded at the stage boundaries.
+int record_substep(cm_body* b, hipEvent_t* ev = nullptr) {
   hipStream_t s = b->stream;
   const cm_params& P = b->prm;
   PbdBufs pb = pbd_bufs(b);
   PbdParams pp = pbd_params(b);
+  auto mark = [&](int k) {
+    if (ev) (void)hipEventRecord(ev[k], s);
+  };
+  mark(CM_STAGE_PREDICT);
   int rc = launch_predict(pb, pp, s);
   if (rc) return rc;
   const bool coll = (P.flags & CM_F_SELF_COLLISION) && b->t >= 2;
+  mark(CM_STAGE_TRIANGLES);
   if (coll) {
     rc = launch_tri_update(b->n, b->t, b->x, b->p, b->tri, P.thickness, P.morton_bits,
                            P.morton_normalize, b->lb.amin, b->lb.amax, b->keys, b->perm, b->cen,
                            b->box, s);
     if (rc) return rc;
+  }
+  mark(CM_STAGE_SORT);
+  if (coll) {
     rc = radix_sort(b->keys, b->perm, b->t, P.morton_bits == 30 ? 30 : 63, b->sort_temp, s);
     if (rc) return rc;
     rc = launch_gather_tris(b->t, b->perm, b->tri, b->tri_alt,
                             (P.ref_compat & CM_RC_STALE_LEAF_AABB) ? 0 : 1, b->lb.amin,
                             b->lb.amax, b->tmp_min, b->tmp_max, s);
     if (rc) return rc;
+  }
+  mark(CM_STAGE_TREE);
+  if (coll) {
     rc = launch_build(b->lb, b->keys, (P.ref_compat & CM_RC_MIDPOINT_SPLIT) ? 1 : 0, s);
     if (rc) return rc;
     rc = launch_refit(b->lb, s);
     if (rc) return rc;
+  }
+  mark(CM_STAGE_PAIRS);
+  if (coll) {
     rc = launch_pairs(b->lb, b->cap, b->chunk, b->leaf_cnt, b->leaf_off, b->pairs, b->counters,
                       b->scan_temp, s);
     if (rc) return rc;
+  }
+  mark(CM_STAGE_NARROW);
+  if (coll) {
     rc = launch_narrow(b->nb, b->tri_alt, b->x, b->p, b->v, P.thickness, P.timestep,
                        (P.flags & CM_F_EE_CONTACTS) ? 1 : 0, s);
     if (rc) return rc;
@@ -352,9 +370,13 @@ int record_substep(cm_body* b) {
   }
   NarrowBufs nb = b->nb;
   if (!coll) nb.dprefix = nullptr;
+  mark(CM_STAGE_SOLVER);
   rc = launch_solver(pb, pp, nb, s);
   if (rc) return rc;
-  return launch_integrate(pb, pp, s);
+  mark(CM_STAGE_INTEGRATE);
+  rc = launch_integrate(pb, pp, s);
+  mark(CM_NUM_STAGES);
+  return rc;
 }
 
 int ensure_graph(cm_body* b) {
@@ -525,6 +547,7 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = dalloc(b, &b->lb.rmlr, t)) || (rc = dalloc(b, &b->lb.claim_cnt, m)) ||
       (rc = dalloc(b, &b->lb.claim_hi, m)) || (rc = dalloc(b, &b->lb.claim_lo, m)) ||
       (rc = dalloc(b, &b->lb.visited, t)) || (rc = dalloc(b, &b->lb.done, t)) ||
+      (rc = dalloc(b, &b->lb.ranges, t)) || (rc = dalloc(b, &b->lb.nodes, 4 * (size_t)t)) ||
       (rc = dalloc(b, &b->chunk, pcap)) || (rc = dalloc(b, &b->pairs, pcap)) ||
       (rc = dalloc(b, &b->leaf_cnt, t)) || (rc = dalloc(b, &b->leaf_off, t)) ||
       (rc = dalloc(b, (char**)&b->scan_temp, scan_temp_bytes(0))))
@@ -536,15 +559,19 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   CM_HIP(hipMemset(b->lb.rmlr, 0, sizeof(int) * t));
   CM_HIP(hipMemset(b->keys, 0, sizeof(uint64_t) * t));
   CM_HIP(hipMemset(b->perm, 0, sizeof(uint32_t) * t));
-  // narrow phase: candidate capacities per triangle (reserved[0] / [1] override)
-  const int64_t vf_per = params->reserved[0] > 0 ? params->reserved[0] : 24;
-  const int64_t ee_per = params->reserved[1] > 0 ? params->reserved[1] : 48;
+  // narrow phase capacities: candidates per triangle (reserved[0] / [1] override the
+  // VF / EE budgets), CCD survivors per triangle (reserved[2])
+  const int64_t vf_per = params->reserved[0] > 0 ? params->reserved[0] : 128;
+  const int64_t ee_per = params->reserved[1] > 0 ? params->reserved[1] : 192;
+  const int64_t out_per = params->reserved[2] > 0 ? params->reserved[2] : 16;
   NarrowBufs& nb = b->nb;
   nb.n = n;
   nb.t = t;
   nb.pair_cap = pcap;
   nb.vf_cap = std::min<int64_t>(vf_per * t + 1024, 6 * pcap);
   nb.ee_cap = std::min<int64_t>(ee_per * t + 1024, 9 * pcap);
+  nb.vf_out_cap = std::min<int64_t>(out_per * t + 65536, nb.vf_cap);
+  nb.ee_out_cap = std::min<int64_t>(out_per * t + 65536, nb.ee_cap);
   nb.pairs = b->pairs;
   nb.d_npairs = b->counters + 0;
   nb.d_ncand_vf = b->counters + 1;
@@ -557,15 +584,12 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   if ((rc = dalloc(b, &nb.mask, pcap)) || (rc = dalloc(b, &nb.cnt_vf, pcap)) ||
       (rc = dalloc(b, &nb.cnt_ee, pcap)) || (rc = dalloc(b, &nb.cand_vf, nb.vf_cap)) ||
       (rc = dalloc(b, &nb.cand_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.flag_vf, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.stage_w_vf, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.stage_tn_vf, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.stage_w_ee, nb.ee_cap)) ||
-      (rc = dalloc(b, &nb.stage_tn_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.vf_v, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.vf_w, nb.vf_cap)) || (rc = dalloc(b, &nb.vf_tn, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.ee_v, nb.ee_cap)) || (rc = dalloc(b, &nb.ee_w, nb.ee_cap)) ||
-      (rc = dalloc(b, &nb.ee_tn, nb.ee_cap)) || (rc = dalloc(b, &nb.dcount, n)) ||
+      (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.vf_v, nb.vf_out_cap)) ||
+      (rc = dalloc(b, &nb.vf_w, nb.vf_out_cap)) || (rc = dalloc(b, &nb.vf_tn, nb.vf_out_cap)) ||
+      (rc = dalloc(b, &nb.ee_v, nb.ee_out_cap)) || (rc = dalloc(b, &nb.ee_w, nb.ee_out_cap)) ||
+      (rc = dalloc(b, &nb.ee_tn, nb.ee_out_cap)) || (rc = dalloc(b, &nb.dcount, n)) ||
       (rc = dalloc(b, &nb.dprefix, (size_t)n + 1)) || (rc = dalloc(b, &nb.dcursor, n)) ||
-      (rc = dalloc(b, &nb.drec, 4 * (size_t)(nb.vf_cap + nb.ee_cap))))
+      (rc = dalloc(b, &nb.drec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))))
     return fail(rc);
   CM_HIP(hipMemset(nb.dprefix, 0, sizeof(int) * ((size_t)n + 1)));
   if (params->flags & CM_F_SPHERE) {
@@ -845,6 +869,30 @@ int cm_time_kernel(cm_body* b, int32_t which, int32_t reps, float* avg_ms) {
   if (rc) return rc;
   *avg_ms = ms / (float)reps;
   return CM_OK;
+}
+
+int cm_stage_times(cm_body* b, int32_t substeps, float* ms) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (substeps < 1 || !ms) {
+    set_error("cm_stage_times: invalid argument");
+    return CM_ERR_ARG;
+  }
+  hipEvent_t ev[CM_NUM_STAGES + 1];
+  for (int k = 0; k <= CM_NUM_STAGES; ++k) CM_HIP(hipEventCreate(&ev[k]));
+  for (int k = 0; k < CM_NUM_STAGES; ++k) ms[k] = 0.f;
+  for (int s = 0; s < substeps && !rc; ++s) {
+    rc = record_substep(b, ev);
+    if (rc) break;
+    CM_HIP(hipEventSynchronize(ev[CM_NUM_STAGES]));
+    for (int k = 0; k < CM_NUM_STAGES; ++k) {
+      float m = 0.f;
+      CM_HIP(hipEventElapsedTime(&m, ev[k], ev[k + 1]));
+      ms[k] += m;
+    }
+  }
+  for (int k = 0; k <= CM_NUM_STAGES; ++k) (void)hipEventDestroy(ev[k]);
+  return rc;
 }
 
 int cm_query_pairs(cm_body* b, int32_t* pairs, int32_t capacity, int32_t* n_pairs) {

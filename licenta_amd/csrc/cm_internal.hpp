@@ -62,6 +62,8 @@ struct LbvhBufs {
   int *claim_cnt = nullptr, *claim_hi = nullptr, *claim_lo = nullptr;  // 2T-1
   int* visited = nullptr;                                              // T-1
   int* done = nullptr;                                                 // T-1 (fallback)
+  int2* ranges = nullptr;                                              // T-1 key ranges
+  float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
   uint32_t* warn = nullptr;
 };
 size_t lbvh_build_temp_bytes(int t);
@@ -71,6 +73,7 @@ int launch_tri_update(int n, int t, const float4* x, const float4* p, const int4
                       uint32_t* vals, float4* centroid_tmp, float4* box_tmp, hipStream_t s);
 int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStream_t s);
 int launch_refit(const LbvhBufs& b, hipStream_t s);
+int launch_pack_nodes(const LbvhBufs& b, hipStream_t s);
 // pairs: traversal into chunk (T*cap int2) + per-leaf counts, then compaction.
 int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
                  int2* pairs, int* d_npairs, void* scan_temp, hipStream_t s);
@@ -83,7 +86,8 @@ int launch_copy_i4(int64_t n, const int4* src, int4* dst, hipStream_t s);
 struct NarrowBufs {
   int n = 0, t = 0;
   int64_t pair_cap = 0;       // capacity of pairs (T*cap)
-  int64_t vf_cap = 0, ee_cap = 0;
+  int64_t vf_cap = 0, ee_cap = 0;          // candidate capacities
+  int64_t vf_out_cap = 0, ee_out_cap = 0;  // contact (CCD survivor) capacities
   const int2* pairs = nullptr;
   const int* d_npairs = nullptr;
   uint16_t* mask = nullptr;   // per pair
@@ -91,8 +95,6 @@ struct NarrowBufs {
   int4 *cand_vf = nullptr, *cand_ee = nullptr;
   int *d_ncand_vf = nullptr, *d_ncand_ee = nullptr;
   int *flag_vf = nullptr, *flag_ee = nullptr;  // per candidate (scanned in place)
-  float4 *stage_w_vf = nullptr, *stage_tn_vf = nullptr, *stage_w_ee = nullptr,
-         *stage_tn_ee = nullptr;
   int4 *vf_v = nullptr, *ee_v = nullptr;      // compacted contacts
   float4 *vf_w = nullptr, *vf_tn = nullptr, *ee_w = nullptr, *ee_tn = nullptr;
   int *d_nvf = nullptr, *d_nee = nullptr;

@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restr
                                                   int midpoint, int* __restrict__ lefts,
                                                   int* __restrict__ rights, int* __restrict__ cnt,
                                                   int* __restrict__ hi, int* __restrict__ lo,
+                                                  int2* __restrict__ ranges,
                                                   uint32_t* __restrict__ warn) {
   const int id = blockIdx.x * kBlock + threadIdx.x;
   const int nin = t - 1;
@@ -161,6 +162,7 @@ __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restr
     if (delta(k, id, id + (l + s) * d, nin) > dmin) l += s;
   const int j = id + l * d;
   const int first = min(id, j), last = max(id, j);
+  ranges[id] = make_int2(first, last);
   // split (Collision/CudaBVH.cu:214-242)
   const uint64_t fc = k[first], lc = k[last];
   int split;
@@ -203,64 +205,157 @@ __global__ void __launch_bounds__(kBlock) k_parents(int t, const int* __restrict
   if (k > 2) atomicOr(warn, kRefitOverflow);
 }
 
-// ---- bottom-up refit through every claimant ---------------------------------
-__device__ __forceinline__ void union_node(int q, int nin, const int* __restrict__ lefts,
-                                           const int* __restrict__ rights,
-                                           float4* __restrict__ amin,
-                                           float4* __restrict__ amax, int* __restrict__ rmll,
-                                           int* __restrict__ rmlr) {
-  const int l = lefts[q], r = rights[q];
-  const float4 a0 = amin[l], a1 = amin[r], b0 = amax[l], b1 = amax[r];
-  amin[q] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z), 0.f);
-  amax[q] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), 0.f);
-  rmll[q] = l >= nin ? l : max(rmll[l], rmlr[l]);
-  rmlr[q] = r >= nin ? r : max(rmll[r], rmlr[r]);
+// ---- bottom-up refit ------------------------------------------------------------
+// Packed node record read by the traversal, 4 float4 per internal node:
+//   [0] left, right, rmll, rmlr (int bits)   [1] left min xyz, left max x
+//   [2] left max yz, right min xy             [3] right min z, right max xyz
+__device__ __forceinline__ void write_node(float4* __restrict__ nodes, int q, int l, int r,
+                                           int a, int b, f3 lmn, f3 lmx, f3 rmn, f3 rmx) {
+  float4* o = nodes + 4 * (int64_t)q;
+  o[0] = make_float4(__int_as_float(l), __int_as_float(r), __int_as_float(a), __int_as_float(b));
+  o[1] = make_float4(lmn.x, lmn.y, lmn.z, lmx.x);
+  o[2] = make_float4(lmx.y, lmx.z, rmn.x, rmn.y);
+  o[3] = make_float4(rmn.z, rmx.x, rmx.y, rmx.z);
 }
 
-constexpr int kClimbStack = 8;
+// node q = union of its two children (Collision/CudaBVH.cu:140-144), all in global memory
+__device__ __forceinline__ void union_global(int q, int nin, const int* __restrict__ lefts,
+                                             const int* __restrict__ rights,
+                                             float4* __restrict__ amin, float4* __restrict__ amax,
+                                             int* __restrict__ rmll, int* __restrict__ rmlr,
+                                             float4* __restrict__ nodes) {
+  const int l = lefts[q], r = rights[q];
+  const f3 lmn = xyz(amin[l]), lmx = xyz(amax[l]), rmn = xyz(amin[r]), rmx = xyz(amax[r]);
+  amin[q] = f4(vmin(lmn, rmn), 0.f);
+  amax[q] = f4(vmax(lmx, rmx), 0.f);
+  const int a = l >= nin ? l : max(rmll[l], rmlr[l]);
+  const int b = r >= nin ? r : max(rmll[r], rmlr[r]);
+  rmll[q] = a;
+  rmlr[q] = b;
+  write_node(nodes, q, l, r, a, b, lmn, lmx, rmn, rmx);
+}
 
-__global__ void __launch_bounds__(kBlock) k_refit(int t, const int* __restrict__ lefts,
-                                                  const int* __restrict__ rights,
-                                                  const int* __restrict__ cnt,
-                                                  const int* __restrict__ hi,
-                                                  const int* __restrict__ lo,
-                                                  int* __restrict__ visited,
-                                                  float4* __restrict__ amin,
-                                                  float4* __restrict__ amax,
-                                                  int* __restrict__ rmll, int* __restrict__ rmlr,
-                                                  uint32_t* __restrict__ warn) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const int nin = t - 1;
-  if (i >= t) return;
-  int stk[kClimbStack];
-  int top = 0;
-  stk[top++] = nin + i;
-  int guard = 0;
-  while (top > 0) {
-    if (++guard > 2 * t) {
-      atomicOr(warn, (uint32_t)CM_WARN_REFIT_LIMIT);
-      return;
-    }
-    const int c = stk[--top];
-    const int k = cnt[c];
-    if (k == 0) continue;
-    for (int s = 0; s < 2; ++s) {
-      if (s == 1 && k < 2) break;
-      const int q = s == 0 ? hi[c] : lo[c];
-      if (s == 1 && q == hi[c]) break;
-      if (atomic_add_acq_rel(&visited[q], 1) == 1) {
-        union_node(q, nin, lefts, rights, amin, amax, rmll, rmlr);
-        if (top < kClimbStack) {
-          stk[top++] = q;
+constexpr int kRefitCH = 1024;  // leaves per refit block
+constexpr int kClimbStack = 4;
+
+// One block per chunk of 1024 consecutive leaves. A node whose key range lies in
+// the chunk ("local") is finished in LDS with workgroup-scope arrival counters;
+// arrivals at nodes whose range crosses the chunk ("escapes", a few per chunk per
+// tree level) then climb with device-scope counters. Every claimant of a node is
+// notified, so a node shared by two parents (midpoint split on tied codes) is still
+// the union of its children; more than two claimants sets kRefitOverflow and the
+// single-block fallback recomputes the tree.
+__global__ void __launch_bounds__(kRefitCH) k_refit_chunk(
+    int t, const int* __restrict__ lefts, const int* __restrict__ rights,
+    const int2* __restrict__ ranges, const int* __restrict__ cnt, const int* __restrict__ hi,
+    const int* __restrict__ lo, int* __restrict__ visited, float4* __restrict__ amin,
+    float4* __restrict__ amax, int* __restrict__ rmll, int* __restrict__ rmlr,
+    float4* __restrict__ nodes, uint32_t* __restrict__ warn) {
+  __shared__ float4 smin[kRefitCH], smax[kRefitCH];
+  __shared__ int sarr[kRefitCH], srml[kRefitCH], srmr[kRefitCH];
+  __shared__ int2 esc[2 * kRefitCH];
+  __shared__ int esc_n;
+  const int nin = t - 1, base = blockIdx.x * kRefitCH, tid = threadIdx.x;
+  sarr[tid] = 0;
+  if (tid == 0) esc_n = 0;
+  __syncthreads();
+  auto is_local = [&](int q) {
+    if (q >= nin) return false;
+    const int2 rg = ranges[q];
+    return rg.x >= base && rg.y < base + kRefitCH;
+  };
+  uint32_t w = 0;
+  const int leaf = base + tid;
+  if (leaf < t) {
+    int stk[kClimbStack];
+    int top = 0;
+    stk[top++] = nin + leaf;
+    int guard = 0;
+    while (top > 0) {
+      if (++guard > 4 * kRefitCH) {
+        w |= kRefitOverflow;
+        break;
+      }
+      const int x = stk[--top];
+      const int k = cnt[x];
+      for (int s = 0; s < 2 && s < k; ++s) {
+        const int q = s == 0 ? hi[x] : lo[x];
+        if (s == 1 && q == hi[x]) break;
+        if (is_local(q)) {
+          const int sq = q - base;
+          if (__hip_atomic_fetch_add(&sarr[sq], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 1) {
+            const int l = lefts[q], r = rights[q];
+            f3 lmn, lmx, rmn, rmx;
+            int a, b;
+            if (l >= nin) {
+              lmn = xyz(amin[l]); lmx = xyz(amax[l]); a = l;
+            } else {
+              lmn = xyz(smin[l - base]); lmx = xyz(smax[l - base]); a = max(srml[l - base], srmr[l - base]);
+            }
+            if (r >= nin) {
+              rmn = xyz(amin[r]); rmx = xyz(amax[r]); b = r;
+            } else {
+              rmn = xyz(smin[r - base]); rmx = xyz(smax[r - base]); b = max(srml[r - base], srmr[r - base]);
+            }
+            smin[sq] = f4(vmin(lmn, rmn), 0.f);
+            smax[sq] = f4(vmax(lmx, rmx), 0.f);
+            srml[sq] = a;
+            srmr[sq] = b;
+            write_node(nodes, q, l, r, a, b, lmn, lmx, rmn, rmx);
+            if (top < kClimbStack) stk[top++] = q; else w |= kRefitOverflow;
+          }
         } else {
-          atomicOr(warn, kRefitOverflow);
+          const int e = atomicAdd(&esc_n, 1);
+          if (e < 2 * kRefitCH) esc[e] = make_int2(x, q); else w |= kRefitOverflow;
+        }
+      }
+      if (k > 2) w |= kRefitOverflow;
+    }
+  }
+  __syncthreads();
+  {
+    const int q = base + tid;
+    if (q < nin && sarr[tid] >= 2 && is_local(q)) {
+      amin[q] = smin[tid];
+      amax[q] = smax[tid];
+      rmll[q] = srml[tid];
+      rmlr[q] = srmr[tid];
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  const int ne = min(esc_n, 2 * kRefitCH);
+  for (int e = tid; e < ne; e += kRefitCH) {
+    int stk[kClimbStack];
+    int top = 0;
+    const int q0 = esc[e].y;
+    if (atomic_add_acq_rel(&visited[q0], 1) == 1) {
+      union_global(q0, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
+      stk[top++] = q0;
+    }
+    int guard = 0;
+    while (top > 0) {
+      if (++guard > 2 * t) {
+        w |= CM_WARN_REFIT_LIMIT;
+        break;
+      }
+      const int x = stk[--top];
+      const int k = cnt[x];
+      if (k > 2) w |= kRefitOverflow;
+      for (int s = 0; s < 2 && s < k; ++s) {
+        const int q = s == 0 ? hi[x] : lo[x];
+        if (s == 1 && q == hi[x]) break;
+        if (atomic_add_acq_rel(&visited[q], 1) == 1) {
+          union_global(q, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
+          if (top < kClimbStack) stk[top++] = q; else w |= kRefitOverflow;
         }
       }
     }
   }
+  if (w) atomicOr(warn, w);
 }
 
-// Fallback when a node has more than two claimants or the climb stack filled:
+// Fallback when a node has more than two claimants or a climb stack filled:
 // one block recomputes every internal node, children first, in passes.
 __global__ void __launch_bounds__(1024) k_refit_fallback(int t, const int* __restrict__ lefts,
                                                          const int* __restrict__ rights,
@@ -269,6 +364,7 @@ __global__ void __launch_bounds__(1024) k_refit_fallback(int t, const int* __res
                                                          float4* __restrict__ amax,
                                                          int* __restrict__ rmll,
                                                          int* __restrict__ rmlr,
+                                                         float4* __restrict__ nodes,
                                                          const uint32_t* __restrict__ warn) {
   if (!(*warn & kRefitOverflow)) return;
   const int nin = t - 1;
@@ -282,7 +378,7 @@ __global__ void __launch_bounds__(1024) k_refit_fallback(int t, const int* __res
       const bool rl = l >= nin || (done[l] != 0 && done[l] < pass);
       const bool rr = r >= nin || (done[r] != 0 && done[r] < pass);
       if (rl && rr) {
-        union_node(q, nin, lefts, rights, amin, amax, rmll, rmlr);
+        union_global(q, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
         done[q] = pass;
         changed = 1;
       }
@@ -292,15 +388,15 @@ __global__ void __launch_bounds__(1024) k_refit_fallback(int t, const int* __res
 }
 
 // ---- traversal ----------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_traverse(int t, const int* __restrict__ lefts,
-                                                     const int* __restrict__ rights,
+constexpr int kLdsStack = 24;  // per-thread stack entries kept in LDS, the rest private
+
+__global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __restrict__ nodes,
                                                      const float4* __restrict__ amin,
-                                                     const float4* __restrict__ amax,
-                                                     const int* __restrict__ rmll,
-                                                     const int* __restrict__ rmlr, int cap,
+                                                     const float4* __restrict__ amax, int cap,
                                                      int2* __restrict__ chunk,
                                                      int* __restrict__ counts,
                                                      uint32_t* __restrict__ warn) {
+  __shared__ int sstk[kLdsStack][kBlock];
   const int leaf = blockIdx.x * kBlock + threadIdx.x;
   if (leaf >= t) return;
   const int nin = t - 1;
@@ -309,48 +405,62 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const int* __restric
   int2* out = chunk + (int64_t)leaf * cap;
   int ncol = 0;
   uint32_t w = 0;
-  int stack[64];
-  stack[0] = -1;
-  int top = 1;
+  int pstk[63 - kLdsStack];
+  int top = 0;  // 63 usable entries, as the reference's int[64] with its -1 sentinel
   int node = 0;
   const long long limit = 64ll * t + 4096;
   long long visits = 0;
-  do {
+  while (true) {
     if (++visits > limit) {
       w |= CM_WARN_TRAVERSAL_LIMIT;
       break;
     }
-    const int l = lefts[node], r = rights[node];
-    bool ol = false, orr = false;
-    if (rmll[node] > me && overlap(mn, mx, xyz(amin[l]), xyz(amax[l]))) ol = true;
-    if (rmlr[node] > me && overlap(mn, mx, xyz(amin[r]), xyz(amax[r]))) orr = true;
+    const float4* nd = nodes + 4 * (int64_t)node;
+    const float4 r0 = nd[0], r1 = nd[1], r2 = nd[2], r3 = nd[3];
+    const int l = __float_as_int(r0.x), r = __float_as_int(r0.y);
+    const int rml = __float_as_int(r0.z), rmr = __float_as_int(r0.w);
+    const bool ol = rml > me && overlap(mn, mx, mk3(r1.x, r1.y, r1.z), mk3(r1.w, r2.x, r2.y));
+    const bool orr = rmr > me && overlap(mn, mx, mk3(r2.z, r2.w, r3.x), mk3(r3.y, r3.z, r3.w));
     if (ol && l >= nin) {
-      if (ncol < cap)
-        out[ncol++] = make_int2(leaf, l - nin);
-      else
-        w |= CM_WARN_PAIR_CAP;
+      if (ncol < cap) out[ncol++] = make_int2(leaf, l - nin); else w |= CM_WARN_PAIR_CAP;
     }
     if (orr && r >= nin) {
-      if (ncol < cap)
-        out[ncol++] = make_int2(leaf, r - nin);
-      else
-        w |= CM_WARN_PAIR_CAP;
+      if (ncol < cap) out[ncol++] = make_int2(leaf, r - nin); else w |= CM_WARN_PAIR_CAP;
     }
     const bool tl = ol && l < nin, tr = orr && r < nin;
     if (!tl && !tr) {
-      node = stack[--top];
+      if (top == 0) break;
+      --top;
+      node = top < kLdsStack ? sstk[top][threadIdx.x] : pstk[top - kLdsStack];
     } else {
       node = tl ? l : r;
       if (tl && tr) {
-        if (top < 64)
-          stack[top++] = r;
-        else
+        if (top < 63) {
+          if (top < kLdsStack) sstk[top][threadIdx.x] = r; else pstk[top - kLdsStack] = r;
+          ++top;
+        } else {
           w |= CM_WARN_STACK_OVERFLOW;
+        }
       }
     }
-  } while (top != 0);
+  }
   counts[leaf] = ncol;
   if (w) atomicOr(warn, w);
+}
+
+// packed node records from separate arrays (unit-level pair query)
+__global__ void __launch_bounds__(kBlock) k_pack_nodes(int t, const int* __restrict__ lefts,
+                                                       const int* __restrict__ rights,
+                                                       const float4* __restrict__ amin,
+                                                       const float4* __restrict__ amax,
+                                                       const int* __restrict__ rmll,
+                                                       const int* __restrict__ rmlr,
+                                                       float4* __restrict__ nodes) {
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= t - 1) return;
+  const int l = lefts[q], r = rights[q];
+  write_node(nodes, q, l, r, rmll[q], rmlr[q], xyz(amin[l]), xyz(amax[l]), xyz(amin[r]),
+             xyz(amax[r]));
 }
 
 // one wave per leaf copies its chunk to the compacted list
@@ -424,7 +534,7 @@ int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStrea
   hipLaunchKernelGGL(k_build_init, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
                      b.claim_hi, b.claim_lo, b.visited);
   hipLaunchKernelGGL(k_build, dim3(nblocks(t - 1)), dim3(kBlock), 0, s, t, keys, midpoint, b.lefts,
-                     b.rights, b.claim_cnt, b.claim_hi, b.claim_lo, b.warn);
+                     b.rights, b.claim_cnt, b.claim_hi, b.claim_lo, b.ranges, b.warn);
   hipLaunchKernelGGL(k_parents, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
                      b.claim_hi, b.parents, b.warn);
   CM_LAUNCH_CHECK();
@@ -434,11 +544,11 @@ int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStrea
 int launch_refit(const LbvhBufs& b, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return CM_OK;
-  hipLaunchKernelGGL(k_refit, dim3(nblocks(t)), dim3(kBlock), 0, s, t, b.lefts, b.rights,
-                     b.claim_cnt, b.claim_hi, b.claim_lo, b.visited, b.amin, b.amax, b.rmll,
-                     b.rmlr, b.warn);
+  hipLaunchKernelGGL(k_refit_chunk, dim3((t + kRefitCH - 1) / kRefitCH), dim3(kRefitCH), 0, s, t,
+                     b.lefts, b.rights, b.ranges, b.claim_cnt, b.claim_hi, b.claim_lo, b.visited,
+                     b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.warn);
   hipLaunchKernelGGL(k_refit_fallback, dim3(1), dim3(1024), 0, s, t, b.lefts, b.rights, b.done,
-                     b.amin, b.amax, b.rmll, b.rmlr, b.warn);
+                     b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.warn);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -447,13 +557,22 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
                  int* d_npairs, void* scan_temp, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return fill_i32(d_npairs, 0, 1, s);
-  hipLaunchKernelGGL(k_traverse, dim3(nblocks(t)), dim3(kBlock), 0, s, t, b.lefts, b.rights,
-                     b.amin, b.amax, b.rmll, b.rmlr, cap, chunk, counts, b.warn);
+  hipLaunchKernelGGL(k_traverse, dim3(nblocks(t)), dim3(kBlock), 0, s, t, b.nodes, b.amin, b.amax,
+                     cap, chunk, counts, b.warn);
   CM_LAUNCH_CHECK();
   int rc = scan_exclusive(counts, offsets, t, nullptr, d_npairs, scan_temp, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_compact_pairs, dim3(nblocks((int64_t)t * 64)), dim3(kBlock), 0, s, t, cap,
                      chunk, counts, offsets, pairs);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_pack_nodes(const LbvhBufs& b, hipStream_t s) {
+  const int t = b.t;
+  if (t < 2) return CM_OK;
+  hipLaunchKernelGGL(k_pack_nodes, dim3(nblocks(t - 1)), dim3(kBlock), 0, s, t, b.lefts, b.rights,
+                     b.amin, b.amax, b.rmll, b.rmlr, b.nodes);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -480,11 +599,13 @@ int launch_copy_i4(int64_t n, const int4* src, int4* dst, hipStream_t s) {
 
 size_t lbvh_build_temp_bytes(int t) {
   const size_t m = t > 0 ? 2 * (size_t)t - 1 : 0;
-  return 3 * align_up(sizeof(int) * m) + 2 * align_up(sizeof(int) * (size_t)(t > 0 ? t : 1));
+  const size_t tt = (size_t)(t > 0 ? t : 1);
+  return 3 * align_up(sizeof(int) * m) + 2 * align_up(sizeof(int) * tt) +
+         align_up(sizeof(int2) * tt) + align_up(4 * sizeof(float4) * tt);
 }
 size_t lbvh_pairs_temp_bytes(int t, int cap) {
   return align_up(sizeof(int2) * (size_t)t * cap) + 2 * align_up(sizeof(int) * (size_t)t) +
-         scan_temp_bytes(t);
+         align_up(4 * sizeof(float4) * (size_t)t) + scan_temp_bytes(t);
 }
 
 }  // namespace cm
@@ -549,6 +670,10 @@ static void carve_build(int t, void* d_temp, LbvhBufs& b) {
   b.visited = (int*)tp;
   tp += align_up(sizeof(int) * (size_t)t);
   b.done = (int*)tp;
+  tp += align_up(sizeof(int) * (size_t)t);
+  b.ranges = (int2*)tp;
+  tp += align_up(sizeof(int2) * (size_t)t);
+  b.nodes = (float4*)tp;
 }
 
 extern "C" int cm_lbvh_build(int32_t t, const uint64_t* d_keys, int32_t midpoint_split,
@@ -624,6 +749,10 @@ extern "C" int cm_lbvh_pairs(int32_t t, const int32_t* d_lefts, const int32_t* d
   tp += align_up(sizeof(int) * (size_t)t);
   int* offsets = (int*)tp;
   tp += align_up(sizeof(int) * (size_t)t);
+  b.nodes = (float4*)tp;
+  tp += align_up(4 * sizeof(float4) * (size_t)t);
+  int rc = launch_pack_nodes(b, (hipStream_t)stream);
+  if (rc) return rc;
   return launch_pairs(b, cap, chunk, counts, offsets, (int2*)d_pairs, d_n_pairs, tp,
                       (hipStream_t)stream);
 }

@@ -276,87 +276,90 @@ __device__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, float th) {
   return false;
 }
 
+// CCD of one candidate (Collision/CudaPrimitiveTests.cu:128-238) on prev positions x
+// and velocities v; h receives the contact data when it hits.
+__device__ bool ccd_candidate(bool isvf, int4 c, const float4* __restrict__ x,
+                              const float4* __restrict__ v, float th, float dt, Hit& h) {
+  h.t = -1.f;
+  h.w[0] = h.w[1] = h.w[2] = h.w[3] = -1.f;
+  h.n = mk3(-1.f, -1.f, -1.f);
+  float times[10];
+  if (isvf) {
+    if (c.x == c.y || c.x == c.z || c.x == c.w) return false;  // VFSanity
+    f3 x4 = xyz(x[c.x]), x1 = xyz(x[c.y]), x2 = xyz(x[c.z]), x3 = xyz(x[c.w]);
+    const f3 v4 = xyz(v[c.x]), v1 = xyz(v[c.y]), v2 = xyz(v[c.z]), v3 = xyz(v[c.w]);
+    const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
+    for (int i = 0; i < nt; ++i) {
+      const float t = times[i] * dt;
+      x4 = add(x4, mul(v4, t));
+      x1 = add(x1, mul(v1, t));
+      x2 = add(x2, mul(v2, t));
+      x3 = add(x3, mul(v3, t));
+      if (test_vf(x4, x1, x2, x3, h, th)) {
+        h.t = t;
+        return true;
+      }
+    }
+    return false;
+  }
+  if (c.x == c.z || c.x == c.w || c.y == c.z || c.y == c.w) return false;  // EESanity
+  f3 x1 = xyz(x[c.x]), x2 = xyz(x[c.y]), x3 = xyz(x[c.z]), x4 = xyz(x[c.w]);
+  const f3 v1 = xyz(v[c.x]), v2 = xyz(v[c.y]), v3 = xyz(v[c.z]), v4 = xyz(v[c.w]);
+  const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
+  for (int i = 0; i < nt; ++i) {
+    const float t = times[i] * dt;
+    x1 = add(x1, mul(v1, t));
+    x2 = add(x2, mul(v2, t));
+    x3 = add(x3, mul(v3, t));
+    x4 = add(x4, mul(v4, t));
+    if (test_ee(x1, x2, x3, x4, h, th)) {
+      h.t = t;
+      return true;
+    }
+  }
+  return false;
+}
+
+// pass 1: hit flags for every candidate (VF then EE)
 __global__ void __launch_bounds__(kBlock) k_ccd(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
     const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
     int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
-    float dt, int* __restrict__ flag_vf, int* __restrict__ flag_ee, float4* __restrict__ w_vf,
-    float4* __restrict__ tn_vf, float4* __restrict__ w_ee, float4* __restrict__ tn_ee) {
+    float dt, int* __restrict__ flag_vf, int* __restrict__ flag_ee) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
   for (int id = blockIdx.x * kBlock + threadIdx.x; id < nvf + nee; id += gridDim.x * kBlock) {
     const bool isvf = id < nvf;
     const int k = isvf ? id : id - nvf;
-    const int4 c = isvf ? cand_vf[k] : cand_ee[k];
-    bool hit = false;
     Hit h;
-    h.t = -1.f;
-    h.w[0] = h.w[1] = h.w[2] = h.w[3] = -1.f;
-    h.n = mk3(-1.f, -1.f, -1.f);
-    float times[10];
-    if (isvf) {
-      if (!(c.x == c.y || c.x == c.z || c.x == c.w)) {  // VFSanity
-        f3 x4 = xyz(x[c.x]), x1 = xyz(x[c.y]), x2 = xyz(x[c.z]), x3 = xyz(x[c.w]);
-        const f3 v4 = xyz(v[c.x]), v1 = xyz(v[c.y]), v2 = xyz(v[c.z]), v3 = xyz(v[c.w]);
-        const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
-        for (int i = 0; i < nt && !hit; ++i) {
-          const float t = times[i] * dt;
-          x4 = add(x4, mul(v4, t));
-          x1 = add(x1, mul(v1, t));
-          x2 = add(x2, mul(v2, t));
-          x3 = add(x3, mul(v3, t));
-          if (test_vf(x4, x1, x2, x3, h, th)) {
-            h.t = t;
-            hit = true;
-          }
-        }
-      }
-      flag_vf[k] = hit;
-      if (hit) {
-        w_vf[k] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
-        tn_vf[k] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
-      }
-    } else {
-      if (!(c.x == c.z || c.x == c.w || c.y == c.z || c.y == c.w)) {  // EESanity
-        f3 x1 = xyz(x[c.x]), x2 = xyz(x[c.y]), x3 = xyz(x[c.z]), x4 = xyz(x[c.w]);
-        const f3 v1 = xyz(v[c.x]), v2 = xyz(v[c.y]), v3 = xyz(v[c.z]), v4 = xyz(v[c.w]);
-        const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
-        for (int i = 0; i < nt && !hit; ++i) {
-          const float t = times[i] * dt;
-          x1 = add(x1, mul(v1, t));
-          x2 = add(x2, mul(v2, t));
-          x3 = add(x3, mul(v3, t));
-          x4 = add(x4, mul(v4, t));
-          if (test_ee(x1, x2, x3, x4, h, th)) {
-            h.t = t;
-            hit = true;
-          }
-        }
-      }
-      flag_ee[k] = hit;
-      if (hit) {
-        w_ee[k] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
-        tn_ee[k] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
-      }
-    }
+    const bool hit = ccd_candidate(isvf, isvf ? cand_vf[k] : cand_ee[k], x, v, th, dt, h);
+    if (isvf) flag_vf[k] = hit; else flag_ee[k] = hit;
   }
 }
 
-// flags were scanned in place into offsets; total in *d_n. Recover each flag as
-// off[k+1]-off[k] (the last one from the total).
-__global__ void __launch_bounds__(kBlock) k_compact_contacts(
-    const int4* __restrict__ cand, const int* __restrict__ d_ncand, int64_t cap,
-    const int* __restrict__ off, const int* __restrict__ d_n, const float4* __restrict__ sw,
-    const float4* __restrict__ stn, int4* __restrict__ ov, float4* __restrict__ ow,
-    float4* __restrict__ otn) {
+// pass 2: flags were scanned in place into offsets (total in *d_n); each hit is
+// recomputed and written at its compacted position, keeping candidate order.
+__global__ void __launch_bounds__(kBlock) k_ccd_write(
+    bool isvf, const int4* __restrict__ cand, const int* __restrict__ d_ncand, int64_t cap,
+    const int* __restrict__ off, const int* __restrict__ d_n, int64_t out_cap,
+    const float4* __restrict__ x, const float4* __restrict__ v, float th, float dt,
+    int4* __restrict__ ov, float4* __restrict__ ow, float4* __restrict__ otn,
+    uint32_t* __restrict__ warn) {
   const int nc = (int)min((int64_t)*d_ncand, cap);
   const int total = *d_n;
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < nc; k += gridDim.x * kBlock) {
     const int o = off[k];
     const int next = k + 1 < nc ? off[k + 1] : total;
     if (next > o) {
-      ov[o] = cand[k];
-      ow[o] = sw[k];
-      otn[o] = stn[k];
+      if (o >= out_cap) {
+        atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
+        continue;
+      }
+      const int4 c = cand[k];
+      Hit h;
+      ccd_candidate(isvf, c, x, v, th, dt, h);
+      ov[o] = c;
+      ow[o] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+      otn[o] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
     }
   }
 }
@@ -441,8 +444,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_ee, nb.ee_cap);
   hipLaunchKernelGGL(k_ccd, dim3(grid_for(nb.vf_cap + nb.ee_cap)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, th, dt,
-                     nb.flag_vf, nb.flag_ee, nb.stage_w_vf, nb.stage_tn_vf, nb.stage_w_ee,
-                     nb.stage_tn_ee);
+                     nb.flag_vf, nb.flag_ee);
   CM_LAUNCH_CHECK();
   rc = scan_exclusive(nb.flag_vf, nb.flag_vf, (int)nb.vf_cap, nb.d_ncand_vf, nb.d_nvf,
                       nb.scan_temp, s);
@@ -450,12 +452,14 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   rc = scan_exclusive(nb.flag_ee, nb.flag_ee, (int)nb.ee_cap, nb.d_ncand_ee, nb.d_nee,
                       nb.scan_temp, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_compact_contacts, dim3(grid_for(nb.vf_cap)), dim3(kBlock), 0, s,
-                     nb.cand_vf, nb.d_ncand_vf, nb.vf_cap, nb.flag_vf, nb.d_nvf, nb.stage_w_vf,
-                     nb.stage_tn_vf, nb.vf_v, nb.vf_w, nb.vf_tn);
-  hipLaunchKernelGGL(k_compact_contacts, dim3(grid_for(nb.ee_cap)), dim3(kBlock), 0, s,
-                     nb.cand_ee, nb.d_ncand_ee, nb.ee_cap, nb.flag_ee, nb.d_nee, nb.stage_w_ee,
-                     nb.stage_tn_ee, nb.ee_v, nb.ee_w, nb.ee_tn);
+  hipLaunchKernelGGL(k_ccd_write, dim3(grid_for(nb.vf_cap)), dim3(kBlock), 0, s, true, nb.cand_vf,
+                     nb.d_ncand_vf, nb.vf_cap, nb.flag_vf, nb.d_nvf, nb.vf_out_cap, x, v, th, dt,
+                     nb.vf_v, nb.vf_w, nb.vf_tn, nb.warn);
+  hipLaunchKernelGGL(k_ccd_write, dim3(grid_for(nb.ee_cap)), dim3(kBlock), 0, s, false, nb.cand_ee,
+                     nb.d_ncand_ee, nb.ee_cap, nb.flag_ee, nb.d_nee, nb.ee_out_cap, x, v, th, dt,
+                     nb.ee_v, nb.ee_w, nb.ee_tn, nb.warn);
+  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_nvf, nb.vf_out_cap);
+  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_nee, nb.ee_out_cap);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -463,7 +467,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
   int rc = fill_i32(nb.dcount, 0, nb.n, s);
   if (rc) return rc;
-  const int gc = grid_for(nb.vf_cap + nb.ee_cap);
+  const int gc = grid_for(nb.vf_out_cap + nb.ee_out_cap);
   hipLaunchKernelGGL(k_dyn_count, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
                      nb.d_nee, nb.dcount);
   CM_LAUNCH_CHECK();

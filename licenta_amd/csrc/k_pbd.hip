@@ -278,7 +278,9 @@ __device__ f3 bend_corr(int4 ids, float2 kpphi0, int role, int abs_mode,
   const T PIf = (T)3.14159265359f;
   if (d == (T)-1) {
     phi = PIf;
-    if (phi == phi0) return mk3(0, 0, 0);
+    // the n/100 push of :1077-1078 is kept for the reference's absolute form only;
+    // the p1-relative variant takes the smooth limit (no correction)
+    if (phi == phi0 || !abs_mode) return mk3(0, 0, 0);
     if (role == 0) return tto(tdiv(n1, (T)100));
     if (role == 1) return tto(tdiv(n2, (T)100));
     return mk3(0, 0, 0);

@@ -1112,7 +1112,10 @@ struct Sim {
     const T PIf = T(3.14159265359f);
     if (d == T(-1)) {
       phi = PIf;
-      if (phi == phi0) return;
+      // the n/100 push of :1077-1078 only matters when phi0 != pi, i.e. outside the
+      // reference's own rest angle; the p1-relative variant takes the limit of the
+      // smooth formula there (sqrt(1 - d^2) = 0: no correction)
+      if (phi == phi0 || !(prm.ref_compat & CM_RC_BEND_ABS)) return;
       c[0] = divs(n1, T(100));
       c[1] = divs(n2, T(100));
       return;
