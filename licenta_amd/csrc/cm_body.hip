@@ -545,7 +545,7 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = dalloc(b, &b->lb.lefts, t)) || (rc = dalloc(b, &b->lb.rights, t)) ||
       (rc = dalloc(b, &b->lb.parents, m)) || (rc = dalloc(b, &b->lb.rmll, t)) ||
       (rc = dalloc(b, &b->lb.rmlr, t)) || (rc = dalloc(b, &b->lb.claim_cnt, m)) ||
-      (rc = dalloc(b, &b->lb.claim_hi, m)) || (rc = dalloc(b, &b->lb.claim_lo, m)) ||
+      (rc = dalloc(b, &b->lb.claim_hi, m)) || (rc = dalloc(b, &b->lb.claim_list, kMaxClaims * m)) ||
       (rc = dalloc(b, &b->lb.visited, t)) || (rc = dalloc(b, &b->lb.done, t)) ||
       (rc = dalloc(b, &b->lb.ranges, t)) || (rc = dalloc(b, &b->lb.nodes, 4 * (size_t)t)) ||
       (rc = dalloc(b, &b->chunk, pcap)) || (rc = dalloc(b, &b->pairs, pcap)) ||

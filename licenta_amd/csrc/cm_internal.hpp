@@ -41,7 +41,8 @@ inline int nblocks(int64_t n, int block = kBlock) {
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
 // warnings raised by kernels (same bits as CM_WARN_*), plus internal bits
-constexpr uint32_t kRefitOverflow = 1u << 16;  // >2 parents on a node: refit fallback
+constexpr uint32_t kRefitOverflow = 1u << 16;  // refit incomplete: single-block fallback
+constexpr int kMaxClaims = 4;  // claimants kept per node for the refit (more -> fallback)
 
 // ---- primitives (k_prims.hip) --------------------------------------------
 size_t scan_temp_bytes(int n_max);
@@ -59,7 +60,8 @@ struct LbvhBufs {
   int t = 0;
   float4 *amin = nullptr, *amax = nullptr;  // 2T-1 (internal nodes first)
   int *lefts = nullptr, *rights = nullptr, *parents = nullptr, *rmll = nullptr, *rmlr = nullptr;
-  int *claim_cnt = nullptr, *claim_hi = nullptr, *claim_lo = nullptr;  // 2T-1
+  int *claim_cnt = nullptr, *claim_hi = nullptr;  // 2T-1: claimants, highest claimant
+  int* claim_list = nullptr;                        // kMaxClaims * (2T-1) claimant ids
   int* visited = nullptr;                                              // T-1
   int* done = nullptr;                                                 // T-1 (fallback)
   int2* ranges = nullptr;                                              // T-1 key ranges

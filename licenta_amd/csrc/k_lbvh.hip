@@ -9,11 +9,11 @@
 //
 // Layout: node AABBs are float4 min / float4 max arrays of 2T-1 entries, internal
 // nodes [0, T-1) then leaves [T-1, 2T-1), as in the reference. Parent writes are
-// made deterministic with claim records (count, highest and lowest claiming
-// node): parents[c] = highest claimant, which is what the reference's last
-// writer in thread order produces; the refit climbs through every claimant so a
-// node shared by two parents (midpoint split on tied codes) is still the union
-// of its two children.
+// made deterministic with claim records (count, highest claimant and the first
+// four claimant ids): parents[c] = highest claimant, which is what the
+// reference's last writer in thread order produces; the refit climbs through
+// every claimant so a node shared by several parents (midpoint split on tied
+// codes) is still the union of its two children.
 #include "cm_internal.hpp"
 
 namespace cm {
@@ -132,21 +132,22 @@ __device__ __forceinline__ int sgn(int a) { return signbit((float)a) ? -1 : 1; }
 
 __global__ void __launch_bounds__(kBlock) k_build_init(int t, int* __restrict__ cnt,
                                                        int* __restrict__ hi,
-                                                       int* __restrict__ lo,
-                                                       int* __restrict__ visited) {
+                                                       int* __restrict__ visited,
+                                                       uint32_t* __restrict__ warn) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 2 * t - 1) {
     cnt[i] = 0;
     hi[i] = -1;
-    lo[i] = 0x7fffffff;
   }
   if (i < t - 1) visited[i] = 0;
+  // the refit-fallback request is per build; the CM_WARN_* bits stay sticky
+  if (i == 0) atomicAnd(warn, ~kRefitOverflow);
 }
 
 __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restrict__ k,
                                                   int midpoint, int* __restrict__ lefts,
                                                   int* __restrict__ rights, int* __restrict__ cnt,
-                                                  int* __restrict__ hi, int* __restrict__ lo,
+                                                  int* __restrict__ hi, int* __restrict__ list,
                                                   int2* __restrict__ ranges,
                                                   uint32_t* __restrict__ warn) {
   const int id = blockIdx.x * kBlock + threadIdx.x;
@@ -185,12 +186,12 @@ __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restr
   const int right = split + 1 == last ? nin + split + 1 : split + 1;
   lefts[id] = left;
   rights[id] = right;
-  int o1 = atomicAdd(&cnt[left], 1);
+  const int o1 = atomicAdd(&cnt[left], 1);
+  if (o1 < kMaxClaims) list[kMaxClaims * (int64_t)left + o1] = id;
   atomicMax(&hi[left], id);
-  atomicMin(&lo[left], id);
-  int o2 = atomicAdd(&cnt[right], 1);
+  const int o2 = atomicAdd(&cnt[right], 1);
+  if (o2 < kMaxClaims) list[kMaxClaims * (int64_t)right + o2] = id;
   atomicMax(&hi[right], id);
-  atomicMin(&lo[right], id);
   if (o1 > 0 || o2 > 0) atomicOr(warn, (uint32_t)CM_WARN_BROKEN_TREE);
 }
 
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_parents(int t, const int* __restrict
   if (c >= 2 * t - 1) return;
   const int k = cnt[c];
   if (k > 0) parents[c] = hi[c];
-  if (k > 2) atomicOr(warn, kRefitOverflow);
+  if (k > kMaxClaims) atomicOr(warn, kRefitOverflow);
 }
 
 // ---- bottom-up refit ------------------------------------------------------------
@@ -236,19 +237,19 @@ __device__ __forceinline__ void union_global(int q, int nin, const int* __restri
 }
 
 constexpr int kRefitCH = 1024;  // leaves per refit block
-constexpr int kClimbStack = 4;
+constexpr int kClimbStack = 8;
 
 // One block per chunk of 1024 consecutive leaves. A node whose key range lies in
 // the chunk ("local") is finished in LDS with workgroup-scope arrival counters;
 // arrivals at nodes whose range crosses the chunk ("escapes", a few per chunk per
 // tree level) then climb with device-scope counters. Every claimant of a node is
-// notified, so a node shared by two parents (midpoint split on tied codes) is still
-// the union of its children; more than two claimants sets kRefitOverflow and the
-// single-block fallback recomputes the tree.
+// notified, so a node shared by several parents (midpoint split on tied codes) is
+// still the union of its children; more than kMaxClaims claimants sets
+// kRefitOverflow and the single-block fallback recomputes the tree.
 __global__ void __launch_bounds__(kRefitCH) k_refit_chunk(
     int t, const int* __restrict__ lefts, const int* __restrict__ rights,
-    const int2* __restrict__ ranges, const int* __restrict__ cnt, const int* __restrict__ hi,
-    const int* __restrict__ lo, int* __restrict__ visited, float4* __restrict__ amin,
+    const int2* __restrict__ ranges, const int* __restrict__ cnt, const int* __restrict__ list,
+    int* __restrict__ visited, float4* __restrict__ amin,
     float4* __restrict__ amax, int* __restrict__ rmll, int* __restrict__ rmlr,
     float4* __restrict__ nodes, uint32_t* __restrict__ warn) {
   __shared__ float4 smin[kRefitCH], smax[kRefitCH];
@@ -278,9 +279,10 @@ __global__ void __launch_bounds__(kRefitCH) k_refit_chunk(
       }
       const int x = stk[--top];
       const int k = cnt[x];
-      for (int s = 0; s < 2 && s < k; ++s) {
-        const int q = s == 0 ? hi[x] : lo[x];
-        if (s == 1 && q == hi[x]) break;
+      if (k > kMaxClaims) w |= kRefitOverflow;
+      const int kk = min(k, kMaxClaims);
+      for (int s = 0; s < kk; ++s) {
+        const int q = list[kMaxClaims * (int64_t)x + s];
         if (is_local(q)) {
           const int sq = q - base;
           if (__hip_atomic_fetch_add(&sarr[sq], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 1) {
@@ -309,7 +311,6 @@ __global__ void __launch_bounds__(kRefitCH) k_refit_chunk(
           if (e < 2 * kRefitCH) esc[e] = make_int2(x, q); else w |= kRefitOverflow;
         }
       }
-      if (k > 2) w |= kRefitOverflow;
     }
   }
   __syncthreads();
@@ -341,10 +342,10 @@ __global__ void __launch_bounds__(kRefitCH) k_refit_chunk(
       }
       const int x = stk[--top];
       const int k = cnt[x];
-      if (k > 2) w |= kRefitOverflow;
-      for (int s = 0; s < 2 && s < k; ++s) {
-        const int q = s == 0 ? hi[x] : lo[x];
-        if (s == 1 && q == hi[x]) break;
+      if (k > kMaxClaims) w |= kRefitOverflow;
+      const int kk = min(k, kMaxClaims);
+      for (int s = 0; s < kk; ++s) {
+        const int q = list[kMaxClaims * (int64_t)x + s];
         if (atomic_add_acq_rel(&visited[q], 1) == 1) {
           union_global(q, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
           if (top < kClimbStack) stk[top++] = q; else w |= kRefitOverflow;
@@ -421,11 +422,15 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __rest
     const int rml = __float_as_int(r0.z), rmr = __float_as_int(r0.w);
     const bool ol = rml > me && overlap(mn, mx, mk3(r1.x, r1.y, r1.z), mk3(r1.w, r2.x, r2.y));
     const bool orr = rmr > me && overlap(mn, mx, mk3(r2.z, r2.w, r3.x), mk3(r3.y, r3.z, r3.w));
+    // Past the cap the reference keeps walking but drops every further pair, so
+    // stopping at the first dropped pair yields the same list.
     if (ol && l >= nin) {
-      if (ncol < cap) out[ncol++] = make_int2(leaf, l - nin); else w |= CM_WARN_PAIR_CAP;
+      if (ncol == cap) { w |= CM_WARN_PAIR_CAP; break; }
+      out[ncol++] = make_int2(leaf, l - nin);
     }
     if (orr && r >= nin) {
-      if (ncol < cap) out[ncol++] = make_int2(leaf, r - nin); else w |= CM_WARN_PAIR_CAP;
+      if (ncol == cap) { w |= CM_WARN_PAIR_CAP; break; }
+      out[ncol++] = make_int2(leaf, r - nin);
     }
     const bool tl = ol && l < nin, tr = orr && r < nin;
     if (!tl && !tr) {
@@ -532,9 +537,9 @@ int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStrea
   const int t = b.t;
   if (t < 2) return CM_OK;
   hipLaunchKernelGGL(k_build_init, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
-                     b.claim_hi, b.claim_lo, b.visited);
+                     b.claim_hi, b.visited, b.warn);
   hipLaunchKernelGGL(k_build, dim3(nblocks(t - 1)), dim3(kBlock), 0, s, t, keys, midpoint, b.lefts,
-                     b.rights, b.claim_cnt, b.claim_hi, b.claim_lo, b.ranges, b.warn);
+                     b.rights, b.claim_cnt, b.claim_hi, b.claim_list, b.ranges, b.warn);
   hipLaunchKernelGGL(k_parents, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
                      b.claim_hi, b.parents, b.warn);
   CM_LAUNCH_CHECK();
@@ -545,7 +550,7 @@ int launch_refit(const LbvhBufs& b, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return CM_OK;
   hipLaunchKernelGGL(k_refit_chunk, dim3((t + kRefitCH - 1) / kRefitCH), dim3(kRefitCH), 0, s, t,
-                     b.lefts, b.rights, b.ranges, b.claim_cnt, b.claim_hi, b.claim_lo, b.visited,
+                     b.lefts, b.rights, b.ranges, b.claim_cnt, b.claim_list, b.visited,
                      b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.warn);
   hipLaunchKernelGGL(k_refit_fallback, dim3(1), dim3(1024), 0, s, t, b.lefts, b.rights, b.done,
                      b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.warn);
@@ -600,7 +605,8 @@ int launch_copy_i4(int64_t n, const int4* src, int4* dst, hipStream_t s) {
 size_t lbvh_build_temp_bytes(int t) {
   const size_t m = t > 0 ? 2 * (size_t)t - 1 : 0;
   const size_t tt = (size_t)(t > 0 ? t : 1);
-  return 3 * align_up(sizeof(int) * m) + 2 * align_up(sizeof(int) * tt) +
+  return 2 * align_up(sizeof(int) * m) + align_up(sizeof(int) * kMaxClaims * m) +
+         2 * align_up(sizeof(int) * tt) +
          align_up(sizeof(int2) * tt) + align_up(4 * sizeof(float4) * tt);
 }
 size_t lbvh_pairs_temp_bytes(int t, int cap) {
@@ -665,8 +671,8 @@ static void carve_build(int t, void* d_temp, LbvhBufs& b) {
   tp += align_up(sizeof(int) * m);
   b.claim_hi = (int*)tp;
   tp += align_up(sizeof(int) * m);
-  b.claim_lo = (int*)tp;
-  tp += align_up(sizeof(int) * m);
+  b.claim_list = (int*)tp;
+  tp += align_up(sizeof(int) * kMaxClaims * m);
   b.visited = (int*)tp;
   tp += align_up(sizeof(int) * (size_t)t);
   b.done = (int*)tp;

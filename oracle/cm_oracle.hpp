@@ -300,23 +300,25 @@ inline int lbvh_leaf_pairs(int t, int leaf, const int32_t* lefts, const int32_t*
     bool ol = false, orr = false;
     if (rmll[node] > me && aabb_overlap(amin[me], amax[me], amin[l], amax[l])) ol = true;
     if (rmlr[node] > me && aabb_overlap(amin[me], amax[me], amin[r], amax[r])) orr = true;
+    // The reference keeps walking past the cap (Collision/CudaBVH.cu:67-93) but
+    // drops every further pair; stopping at the first dropped one gives the same list.
     if (ol && l >= nin) {
-      if (ncol < cap) {
-        out[2 * ncol] = leaf;
-        out[2 * ncol + 1] = l - nin;
-        ++ncol;
-      } else if (capped) {
-        *capped = true;
+      if (ncol == cap) {
+        if (capped) *capped = true;
+        break;
       }
+      out[2 * ncol] = leaf;
+      out[2 * ncol + 1] = l - nin;
+      ++ncol;
     }
     if (orr && r >= nin) {
-      if (ncol < cap) {
-        out[2 * ncol] = leaf;
-        out[2 * ncol + 1] = r - nin;
-        ++ncol;
-      } else if (capped) {
-        *capped = true;
+      if (ncol == cap) {
+        if (capped) *capped = true;
+        break;
       }
+      out[2 * ncol] = leaf;
+      out[2 * ncol + 1] = r - nin;
+      ++ncol;
     }
     bool tl = ol && l < nin, tr = orr && r < nin;
     if (!tl && !tr) {
