@@ -176,7 +176,9 @@ int cm_set_sphere(cm_body* body, const float center[3], float radius);
  * tri: 4T ints (current triangle order with feature bits) or NULL. */
 int cm_set_state(cm_body* body, const float* x, const float* p, const float* v,
                  const int32_t* tri);
-int cm_get_counts(cm_body* body, int64_t* counts /* [n, t, s, bd, pairs, vf, ee] */);
+/* counts[9] = n, t, s, bd, pairs, vf contacts, ee contacts, vf candidates,
+ * ee candidates (the last substep's; candidates are the CCD inputs) */
+int cm_get_counts(cm_body* body, int64_t* counts);
 int cm_get_buffer(cm_body* body, int32_t which, void* host, int64_t capacity_bytes,
                   int64_t* count);
 int cm_get_warnings(cm_body* body, uint32_t* warnings);

@@ -113,9 +113,10 @@ class ClothBody:
         check(lib().cm_set_state(self.h, ptr(x), ptr(p), ptr(v), ptr(tri)))
 
     def counts(self) -> dict:
-        out = np.zeros(8, dtype=np.int64)
+        out = np.zeros(9, dtype=np.int64)
         check(lib().cm_get_counts(self.h, ptr(out)))
-        return dict(zip(("n", "t", "s", "b", "pairs", "vf", "ee"), out[:7].tolist()))
+        return dict(zip(("n", "t", "s", "b", "pairs", "vf", "ee", "vf_cand", "ee_cand"),
+                        out.tolist()))
 
     def warnings(self) -> int:
         w = C.c_uint32()

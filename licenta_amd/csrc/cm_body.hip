@@ -587,9 +587,11 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.vf_v, nb.vf_out_cap)) ||
       (rc = dalloc(b, &nb.vf_w, nb.vf_out_cap)) || (rc = dalloc(b, &nb.vf_tn, nb.vf_out_cap)) ||
       (rc = dalloc(b, &nb.ee_v, nb.ee_out_cap)) || (rc = dalloc(b, &nb.ee_w, nb.ee_out_cap)) ||
-      (rc = dalloc(b, &nb.ee_tn, nb.ee_out_cap)) || (rc = dalloc(b, &nb.dcount, n)) ||
+      (rc = dalloc(b, &nb.ee_tn, nb.ee_out_cap)) || (rc = dalloc(b, &nb.dcount, (size_t)n + 1)) ||
       (rc = dalloc(b, &nb.dprefix, (size_t)n + 1)) || (rc = dalloc(b, &nb.dcursor, n)) ||
-      (rc = dalloc(b, &nb.drec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))))
+      (rc = dalloc(b, &nb.drec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
+      (rc = dalloc(b, &nb.dtmp, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
+      (rc = dalloc(b, &nb.dlong, (size_t)n)))
     return fail(rc);
   CM_HIP(hipMemset(nb.dprefix, 0, sizeof(int) * ((size_t)n + 1)));
   if (params->flags & CM_F_SPHERE) {
@@ -755,6 +757,8 @@ int cm_get_counts(cm_body* b, int64_t* counts) {
   counts[4] = c[0];
   counts[5] = c[3];
   counts[6] = c[4];
+  counts[7] = c[1];
+  counts[8] = c[2];
   return CM_OK;
 }
 

@@ -101,8 +101,10 @@ struct NarrowBufs {
   float4 *vf_w = nullptr, *vf_tn = nullptr, *ee_w = nullptr, *ee_tn = nullptr;
   int *d_nvf = nullptr, *d_nee = nullptr;
   // per-vertex dynamic record CSR
-  int *dcount = nullptr, *dprefix = nullptr, *dcursor = nullptr;
-  uint32_t* drec = nullptr;  // 4 * (vf_cap + ee_cap)
+  int *dcount = nullptr, *dprefix = nullptr, *dcursor = nullptr;  // n + 1, n + 1, n
+  uint32_t* drec = nullptr;  // 4 * (vf_out_cap + ee_out_cap)
+  uint32_t* dtmp = nullptr;  // same size: staging for very long record lists
+  int* dlong = nullptr;      // n: vertices whose record lists are ordered by a workgroup
   int* d_nrec = nullptr;
   uint32_t* warn = nullptr;
   void* scan_temp = nullptr;

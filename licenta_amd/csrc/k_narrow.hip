@@ -116,22 +116,31 @@ __device__ __forceinline__ float triple_scalar(f3 x21, f3 x31, f3 x41, f3 v21, f
   return dot(cross(add(x21, mul(v21, t)), add(x31, mul(v31, t))), add(x41, mul(v41, t)));
 }
 
-__device__ float secant(f3 x21, f3 x31, f3 x41, f3 v21, f3 v31, f3 v41, float a, float b) {
+// Secant search on [a, b] (Physics/CudaCubicSolver.cu:131-153, 20 steps). The
+// reference evaluates f at both points every step; f(x0) of a step is f(x1) of the
+// previous one, computed from the same inputs, so it is carried over unchanged.
+__device__ __forceinline__ float secant(f3 x21, f3 x31, f3 x41, f3 v21, f3 v31, f3 v41, float a, float b) {
   float x0 = a, x1 = b;
+  float f0 = triple_scalar(x21, x31, x41, v21, v31, v41, x0);
   for (int i = 0; i < 20; ++i) {
-    const float f0 = triple_scalar(x21, x31, x41, v21, v31, v41, x0);
     const float f1 = triple_scalar(x21, x31, x41, v21, v31, v41, x1);
     const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
     if (fabsf(xc - x1) < kCubicEps) return xc;
     x0 = x1;
+    f0 = f1;
     x1 = xc;
   }
   return -15.f;
 }
 
-__device__ int coplanarity_times(f3 x1, f3 x2, f3 x3, f3 x4, f3 v1, f3 v2, f3 v3, f3 v4,
-                                 float* times) {
-  int n = 0;
+// Coplanarity times in [0, 1] (Physics/CudaCubicSolver.cu:5-153), handed to
+// on_root in the reference's order; stops at the first root on_root accepts.
+// The reference collects every root first and then tests them in order until one
+// hits, so stopping early gives the same contact. f at an interval end is shared
+// with the next interval's start.
+template <class OnRoot>
+__device__ __forceinline__ bool coplanarity_times(f3 x1, f3 x2, f3 x3, f3 x4, f3 v1, f3 v2,
+                                                  f3 v3, f3 v4, OnRoot&& on_root) {
   const f3 x21 = sub(x2, x1), x31 = sub(x3, x1), x41 = sub(x4, x1);
   const f3 v21 = sub(v2, v1), v31 = sub(v3, v1), v41 = sub(v4, v1);
   const f3 v21v31 = cross(v21, v31), x21v31 = cross(x21, v31), v21x31 = cross(v21, x31),
@@ -158,23 +167,21 @@ __device__ int coplanarity_times(f3 x1, f3 x2, f3 x3, f3 x4, f3 v1, f3 v2, f3 v3
     if (between(t1, 0.f, 1.f)) { cp1 = t1; have1 = true; ++cc; }
   }
   if (cc == 1 && !have1) cp1 = cp2;
-  float ends[4];
-  int ne = 0;
-  ends[ne++] = 0.f;
-  if (cc >= 1) ends[ne++] = cp1;
-  if (cc == 2) ends[ne++] = cp2;
-  ends[ne++] = 1.f;
-  for (int i = 0; i < ne - 1; ++i) {
-    const float ea = ends[i], eb = ends[i + 1];
-    const float fa = triple_scalar(x21, x31, x41, v21, v31, v41, ea);
+  // interval ends: 0, [cp1], [cp2], 1
+  float ea = 0.f;
+  float fa = triple_scalar(x21, x31, x41, v21, v31, v41, ea);
+  for (int i = 0; i <= cc; ++i) {
+    const float eb = i == cc ? 1.f : (i == 0 ? cp1 : cp2);
     const float fb = triple_scalar(x21, x31, x41, v21, v31, v41, eb);
-    if (fabsf(fa) < kCubicEps) times[n++] = ea;
+    if (fabsf(fa) < kCubicEps && on_root(ea)) return true;
     if (fa * fb < 0.f) {
       const float tt = secant(x21, x31, x41, v21, v31, v41, ea, eb);
-      if (tt != -15) times[n++] = tt;
+      if (tt != -15 && on_root(tt)) return true;
     }
+    ea = eb;
+    fa = fb;
   }
-  return n;
+  return false;
 }
 
 struct Hit {
@@ -183,7 +190,7 @@ struct Hit {
   f3 n;
 };
 
-__device__ bool test_vf(f3 x4, f3 x1, f3 x2, f3 x3, Hit& h, float th) {
+__device__ __forceinline__ bool test_vf(f3 x4, f3 x1, f3 x2, f3 x3, Hit& h, float th) {
   const f3 x13 = sub(x1, x3), x23 = sub(x2, x3), x43 = sub(x4, x3);
   const f3 crs = cross(sub(x2, x1), sub(x3, x1));
   const f3 n = normalize(crs);
@@ -219,7 +226,7 @@ __device__ f3 clamp_on_line(f3 p, f3 a, f3 b) {
   return d1 < d2 ? a : b;
 }
 
-__device__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, float th) {
+__device__ __forceinline__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, float th) {
   const f3 x21 = sub(x2, x1), x31 = sub(x3, x1), x43 = sub(x4, x3);
   const f3 crs = cross(x21, x43);
   if (len(crs) < 0.000000001f) {  // TestEdgeDegenerate (Collision/CudaPrimitiveTests.cu:408-425)
@@ -278,46 +285,40 @@ __device__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, float th) {
 
 // CCD of one candidate (Collision/CudaPrimitiveTests.cu:128-238) on prev positions x
 // and velocities v; h receives the contact data when it hits.
-__device__ bool ccd_candidate(bool isvf, int4 c, const float4* __restrict__ x,
+__device__ __forceinline__ bool ccd_candidate(bool isvf, int4 c, const float4* __restrict__ x,
                               const float4* __restrict__ v, float th, float dt, Hit& h) {
   h.t = -1.f;
   h.w[0] = h.w[1] = h.w[2] = h.w[3] = -1.f;
   h.n = mk3(-1.f, -1.f, -1.f);
-  float times[10];
   if (isvf) {
     if (c.x == c.y || c.x == c.z || c.x == c.w) return false;  // VFSanity
     f3 x4 = xyz(x[c.x]), x1 = xyz(x[c.y]), x2 = xyz(x[c.z]), x3 = xyz(x[c.w]);
     const f3 v4 = xyz(v[c.x]), v1 = xyz(v[c.y]), v2 = xyz(v[c.z]), v3 = xyz(v[c.w]);
-    const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
-    for (int i = 0; i < nt; ++i) {
-      const float t = times[i] * dt;
+    // positions advance cumulatively by each root (Collision/CudaPrimitiveTests.cu:164-171)
+    return coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, [&](float root) {
+      const float t = root * dt;
       x4 = add(x4, mul(v4, t));
       x1 = add(x1, mul(v1, t));
       x2 = add(x2, mul(v2, t));
       x3 = add(x3, mul(v3, t));
-      if (test_vf(x4, x1, x2, x3, h, th)) {
-        h.t = t;
-        return true;
-      }
-    }
-    return false;
+      if (!test_vf(x4, x1, x2, x3, h, th)) return false;
+      h.t = t;
+      return true;
+    });
   }
   if (c.x == c.z || c.x == c.w || c.y == c.z || c.y == c.w) return false;  // EESanity
   f3 x1 = xyz(x[c.x]), x2 = xyz(x[c.y]), x3 = xyz(x[c.z]), x4 = xyz(x[c.w]);
   const f3 v1 = xyz(v[c.x]), v2 = xyz(v[c.y]), v3 = xyz(v[c.z]), v4 = xyz(v[c.w]);
-  const int nt = coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, times);
-  for (int i = 0; i < nt; ++i) {
-    const float t = times[i] * dt;
+  return coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, [&](float root) {
+    const float t = root * dt;
     x1 = add(x1, mul(v1, t));
     x2 = add(x2, mul(v2, t));
     x3 = add(x3, mul(v3, t));
     x4 = add(x4, mul(v4, t));
-    if (test_ee(x1, x2, x3, x4, h, th)) {
-      h.t = t;
-      return true;
-    }
-  }
-  return false;
+    if (!test_ee(x1, x2, x3, x4, h, th)) return false;
+    h.t = t;
+    return true;
+  });
 }
 
 // pass 1: hit flags for every candidate (VF then EE)
@@ -393,12 +394,22 @@ __global__ void __launch_bounds__(kBlock) k_dyn_fill(const int4* __restrict__ vf
   }
 }
 
-// per vertex: order its records by record index (insertion sort; lists are short)
+constexpr int kShortList = 32;   // longer record lists are ordered by a workgroup
+constexpr int kLongLds = 4096;   // records a workgroup ranks in LDS
+
+// per vertex: order its records by record index. Short lists: insertion sort by
+// the vertex's thread; longer ones go to a work list for k_dyn_sort_long.
 __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restrict__ prefix,
-                                                     uint32_t* __restrict__ drec) {
+                                                     uint32_t* __restrict__ drec,
+                                                     int* __restrict__ nlong,
+                                                     int* __restrict__ long_list) {
   const int v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= n) return;
   const int b = prefix[v], e = prefix[v + 1];
+  if (e - b > kShortList) {
+    long_list[atomicAdd(nlong, 1)] = v;
+    return;
+  }
   for (int i = b + 1; i < e; ++i) {
     const uint32_t key = drec[i];
     int j = i - 1;
@@ -407,6 +418,42 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restric
       --j;
     }
     drec[j + 1] = key;
+  }
+}
+
+// one workgroup per long list: record indices are unique, so the rank of a record
+// is the number of smaller records in its list
+__global__ void __launch_bounds__(kBlock) k_dyn_sort_long(const int* __restrict__ prefix,
+                                                          const int* __restrict__ nlong,
+                                                          const int* __restrict__ long_list,
+                                                          uint32_t* __restrict__ drec,
+                                                          uint32_t* __restrict__ tmp) {
+  __shared__ uint32_t sl[kLongLds];
+  const int nl = *nlong;
+  for (int w = blockIdx.x; w < nl; w += gridDim.x) {
+    const int v = long_list[w];
+    const int b = prefix[v], k = prefix[v + 1] - b;
+    const uint32_t* src = drec + b;
+    if (k <= kLongLds) {
+      for (int j = threadIdx.x; j < k; j += kBlock) sl[j] = src[j];
+      __syncthreads();
+      for (int j = threadIdx.x; j < k; j += kBlock) {
+        const uint32_t key = sl[j];
+        int r = 0;
+        for (int m = 0; m < k; ++m) r += sl[m] < key;
+        drec[b + r] = key;
+      }
+    } else {
+      for (int j = threadIdx.x; j < k; j += kBlock) {
+        const uint32_t key = src[j];
+        int r = 0;
+        for (int m = 0; m < k; ++m) r += src[m] < key;
+        tmp[b + r] = key;
+      }
+      __syncthreads();
+      for (int j = threadIdx.x; j < k; j += kBlock) drec[b + j] = tmp[b + j];
+    }
+    __syncthreads();
   }
 }
 
@@ -465,7 +512,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
 }
 
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
-  int rc = fill_i32(nb.dcount, 0, nb.n, s);
+  int rc = fill_i32(nb.dcount, 0, (int64_t)nb.n + 1, s);  // dcount[n]: long-list counter
   if (rc) return rc;
   const int gc = grid_for(nb.vf_out_cap + nb.ee_out_cap);
   hipLaunchKernelGGL(k_dyn_count, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
@@ -478,7 +525,9 @@ int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
   hipLaunchKernelGGL(k_dyn_fill, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
                      nb.d_nee, nb.dcursor, nb.drec);
   hipLaunchKernelGGL(k_dyn_sort, dim3(nblocks(nb.n)), dim3(kBlock), 0, s, nb.n, nb.dprefix,
-                     nb.drec);
+                     nb.drec, nb.dcount + nb.n, nb.dlong);
+  hipLaunchKernelGGL(k_dyn_sort_long, dim3(512), dim3(kBlock), 0, s, nb.dprefix, nb.dcount + nb.n,
+                     nb.dlong, nb.drec, nb.dtmp);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

@@ -10,10 +10,11 @@
 //                    Collision/CudaCollisionDetection.cu:529-567)
 //  k_solve        <- one Jacobi iteration of PBDStepSolver (:648-659): ground, LRA,
 //                    static (stretch :986-1022, bend :1024-1147), dynamic (VF :1212-1311,
-//                    EE :1313-1415) corrections and FinalCorrectionStep (:1690-1698),
-//                    fused per vertex: each vertex re-evaluates the constraints it
-//                    belongs to in the reference's slot order, so the per-vertex sums
-//                    are the reference's gathers (:1417-1475) without the slot buffer.
+//                    EE :1313-1415) corrections and FinalCorrectionStep (:1690-1698)
+//                    in one launch: a workgroup evaluates the slots of 64 consecutive
+//                    vertices in parallel into LDS, then each vertex sums its slots in
+//                    the reference's slot order, so the per-vertex sums are the
+//                    reference's gathers (:1417-1475) without the global slot buffer.
 //  k_integrate    <- _FinalUpdate (:1916-1932) + _WorldConstraints
 //                    (Physics/CudaDeformableUtils.cu:260-269)
 //
@@ -373,70 +374,66 @@ struct SolveArgs {
   const float4 *vf_w, *ee_w;
 };
 
+// one static slot's correction: slot word = bend flag (bit 31), role (bits 29-30),
+// constraint index (bits 0-28)
+__device__ __forceinline__ f3 slot_corr(const SolveArgs& a, uint32_t s,
+                                        const float4* __restrict__ pin) {
+  const int idx = (int)(s & 0x1fffffffu);
+  const int role = (int)((s >> 29) & 3u);
+  if (s & 0x80000000u)
+    return a.bend_fp64 ? bend_corr<double>(a.bid[idx], a.b_kpphi0[idx], role, a.abs_mode, pin)
+                       : bend_corr<float>(a.bid[idx], a.b_kpphi0[idx], role, a.abs_mode, pin);
+  return stretch_corr(a.sid[idx], a.s_l0kp[idx], role, pin);
+}
+
+// one dynamic record's correction: record word = 4 * contact + role, VF contacts first
+__device__ __forceinline__ f3 rec_corr(const SolveArgs& a, uint32_t r, int nvf,
+                                       const float4* __restrict__ pin) {
+  const int ci = (int)(r >> 2), role = (int)(r & 3u);
+  return ci < nvf ? vf_corr(a.vf_v[ci], a.vf_w[ci], role, a.th, pin)
+                  : ee_corr(a.ee_v[ci - nvf], a.ee_w[ci - nvf], role, a.th, pin);
+}
+
+constexpr int kSolveV = 64;        // vertices per workgroup
+constexpr int kSolveChunk = 2048;  // slot corrections staged in LDS per pass
+
+// One Jacobi iteration over kSolveV consecutive vertices. The workgroup's slots
+// (and then its dynamic records) are evaluated in parallel, kSolveChunk at a time,
+// into LDS; each vertex then adds its own entries in slot order, so every per-vertex
+// sum has the reference's order and rounding.
 __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __restrict__ pin,
                                                   float4* __restrict__ pout) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  const float4 pi = pin[i];
+  __shared__ float cx[kSolveChunk], cy[kSolveChunk], cz[kSolveChunk];
+  const int v0 = blockIdx.x * kSolveV;
+  const int v1 = min(v0 + kSolveV, a.n);
+  const int tid = threadIdx.x;
+  const int i = v0 + tid;
+  const bool mine = tid < kSolveV && i < v1;
+  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
   f3 total = mk3(0, 0, 0);
   int cnt = 0;
-  if (a.ground_on) {
-    const float4 gq = a.ground[i];
-    if (gq.w != 0.f) {
-      const f3 nc = mk3(0.f, 1.f, 0.f);
-      const float cv = dot(sub(xyz(pi), xyz(gq)), nc);
-      if (cv < 0) {
-        total = add(total, mul(nc, -cv));
-        cnt += 1;
+  if (mine) {
+    pi = pin[i];
+    if (a.ground_on) {
+      const float4 gq = a.ground[i];
+      if (gq.w != 0.f) {
+        const f3 nc = mk3(0.f, 1.f, 0.f);
+        const float cv = dot(sub(xyz(pi), xyz(gq)), nc);
+        if (cv < 0) {
+          total = add(total, mul(nc, -cv));
+          cnt += 1;
+        }
       }
     }
-  }
-  if (a.F) {
-    f3 acc = mk3(0, 0, 0);
-    int c = 0;
-    for (int j = 0; j < a.F; ++j) {
-      if (a.lra_flag[(int64_t)i * a.F + j]) {
-        const f3 diff = sub(xyz(pi), xyz(pin[a.lra_fixed[j]]));
-        const float l = len(diff);
-        const f3 nn = divs(diff, l);
-        acc = add(acc, mul(nn, -(l - a.lra_l0[(int64_t)i * a.F + j])));
-        ++c;
-      }
-    }
-    total = add(total, acc);
-    cnt += c;
-  }
-  {
-    const int b = a.vprefix[i], e = a.vprefix[i + 1];
-    f3 acc = mk3(0, 0, 0);
-    for (int k = b; k < e; ++k) {
-      const uint32_t s = a.vslot[k];
-      const int idx = (int)(s & 0x1fffffffu);
-      const int role = (int)((s >> 29) & 3u);
-      f3 c;
-      if (s & 0x80000000u)
-        c = a.bend_fp64 ? bend_corr<double>(a.bid[idx], a.b_kpphi0[idx], role, a.abs_mode, pin)
-                        : bend_corr<float>(a.bid[idx], a.b_kpphi0[idx], role, a.abs_mode, pin);
-      else
-        c = stretch_corr(a.sid[idx], a.s_l0kp[idx], role, pin);
-      acc = add(acc, c);
-    }
-    total = add(total, acc);
-    cnt += e - b;
-  }
-  if (a.dprefix) {
-    const int b = a.dprefix[i], e = a.dprefix[i + 1];
-    if (e > b) {
-      const int nvf = *a.d_nvf;
+    if (a.F) {
       f3 acc = mk3(0, 0, 0);
       int c = 0;
-      for (int k = b; k < e; ++k) {
-        const uint32_t r = a.drec[k];
-        const int ci = (int)(r >> 2), role = (int)(r & 3u);
-        const f3 val = ci < nvf ? vf_corr(a.vf_v[ci], a.vf_w[ci], role, a.th, pin)
-                                : ee_corr(a.ee_v[ci - nvf], a.ee_w[ci - nvf], role, a.th, pin);
-        if (!is_zero(val)) {
-          acc = add(acc, val);
+      for (int j = 0; j < a.F; ++j) {
+        if (a.lra_flag[(int64_t)i * a.F + j]) {
+          const f3 diff = sub(xyz(pi), xyz(pin[a.lra_fixed[j]]));
+          const float l = len(diff);
+          const f3 nn = divs(diff, l);
+          acc = add(acc, mul(nn, -(l - a.lra_l0[(int64_t)i * a.F + j])));
           ++c;
         }
       }
@@ -444,7 +441,85 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
       cnt += c;
     }
   }
-  pout[i] = f4(add(xyz(pi), divs(total, (float)cnt)), pi.w);
+  // static slots
+  {
+    const int s0 = a.vprefix[v0], s1 = a.vprefix[v1];
+    int k = mine ? a.vprefix[i] : 0;
+    const int e = mine ? a.vprefix[i + 1] : 0;
+    f3 acc = mk3(0, 0, 0);
+    for (int c0 = s0; c0 < s1; c0 += kSolveChunk) {
+      const int c1 = min(c0 + kSolveChunk, s1);
+      for (int q = c0 + tid; q < c1; q += kBlock) {
+        const f3 c = slot_corr(a, a.vslot[q], pin);
+        cx[q - c0] = c.x;
+        cy[q - c0] = c.y;
+        cz[q - c0] = c.z;
+      }
+      __syncthreads();
+      const int kend = min(e, c1);
+      // LDS reads batched ahead of the dependent adds; the add order is unchanged
+      for (; k + 4 <= kend; k += 4) {
+        const int q = k - c0;
+        const f3 e0 = mk3(cx[q], cy[q], cz[q]), e1 = mk3(cx[q + 1], cy[q + 1], cz[q + 1]);
+        const f3 e2 = mk3(cx[q + 2], cy[q + 2], cz[q + 2]), e3 = mk3(cx[q + 3], cy[q + 3], cz[q + 3]);
+        acc = add(add(add(add(acc, e0), e1), e2), e3);
+      }
+      for (; k < kend; ++k) acc = add(acc, mk3(cx[k - c0], cy[k - c0], cz[k - c0]));
+      __syncthreads();
+    }
+    if (mine) {
+      total = add(total, acc);
+      cnt += e - (int)a.vprefix[i];
+    }
+  }
+  // dynamic records: only non-zero corrections are summed and counted
+  if (a.dprefix) {
+    const int r0 = a.dprefix[v0], r1 = a.dprefix[v1];
+    if (r1 > r0) {
+      const int nvf = *a.d_nvf;
+      int k = mine ? a.dprefix[i] : 0;
+      const int e = mine ? a.dprefix[i + 1] : 0;
+      const bool any = mine && e > k;
+      f3 acc = mk3(0, 0, 0);
+      int c = 0;
+      for (int c0 = r0; c0 < r1; c0 += kSolveChunk) {
+        const int c1 = min(c0 + kSolveChunk, r1);
+        for (int q = c0 + tid; q < c1; q += kBlock) {
+          const f3 val = rec_corr(a, a.drec[q], nvf, pin);
+          cx[q - c0] = val.x;
+          cy[q - c0] = val.y;
+          cz[q - c0] = val.z;
+        }
+        __syncthreads();
+        const int kend = min(e, c1);
+        for (; k + 4 <= kend; k += 4) {
+          const int q = k - c0;
+          f3 ev[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) ev[u] = mk3(cx[q + u], cy[q + u], cz[q + u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (!is_zero(ev[u])) {
+              acc = add(acc, ev[u]);
+              ++c;
+            }
+        }
+        for (; k < kend; ++k) {
+          const f3 val = mk3(cx[k - c0], cy[k - c0], cz[k - c0]);
+          if (!is_zero(val)) {
+            acc = add(acc, val);
+            ++c;
+          }
+        }
+        __syncthreads();
+      }
+      if (any) {
+        total = add(total, acc);
+        cnt += c;
+      }
+    }
+  }
+  if (mine) pout[i] = f4(add(xyz(pi), divs(total, (float)cnt)), pi.w);
 }
 
 __global__ void __launch_bounds__(kBlock) k_integrate(int n, const float4* __restrict__ pfin,
@@ -524,7 +599,7 @@ static SolveArgs solve_args(const PbdBufs& b, const PbdParams& pp, const NarrowB
 
 // `iters` Jacobi iterations ping-ponging p -> p2 -> p ...
 static int launch_iterations(const PbdBufs& b, const SolveArgs& a, int iters, hipStream_t s) {
-  const int g = nblocks(b.n);
+  const int g = (b.n + kSolveV - 1) / kSolveV;
   float4* src = b.p;
   float4* dst = b.p2;
   for (int it = 0; it < iters; ++it) {
