@@ -62,6 +62,7 @@ struct cm_body {
   int2 *chunk = nullptr, *pairs = nullptr;
   int *leaf_cnt = nullptr, *leaf_off = nullptr;
   int* counters = nullptr;  // [0] npairs [1] ncand_vf [2] ncand_ee [3] nvf [4] nee
+  void* lb_temp = nullptr;  // LBVH build/refit scratch (carve_build)
   void* scan_temp = nullptr;
   NarrowBufs nb;
   uint32_t* warn = nullptr;
@@ -544,14 +545,14 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = dalloc(b, &b->lb.amin, m)) || (rc = dalloc(b, &b->lb.amax, m)) ||
       (rc = dalloc(b, &b->lb.lefts, t)) || (rc = dalloc(b, &b->lb.rights, t)) ||
       (rc = dalloc(b, &b->lb.parents, m)) || (rc = dalloc(b, &b->lb.rmll, t)) ||
-      (rc = dalloc(b, &b->lb.rmlr, t)) || (rc = dalloc(b, &b->lb.claim_cnt, m)) ||
-      (rc = dalloc(b, &b->lb.claim_hi, m)) || (rc = dalloc(b, &b->lb.claim_list, kMaxClaims * m)) ||
-      (rc = dalloc(b, &b->lb.visited, t)) || (rc = dalloc(b, &b->lb.done, t)) ||
-      (rc = dalloc(b, &b->lb.ranges, t)) || (rc = dalloc(b, &b->lb.nodes, 4 * (size_t)t)) ||
+      (rc = dalloc(b, &b->lb.rmlr, t)) ||
+      (rc = dalloc(b, (char**)&b->lb_temp, lbvh_build_temp_bytes(t))) ||
       (rc = dalloc(b, &b->chunk, pcap)) || (rc = dalloc(b, &b->pairs, pcap)) ||
       (rc = dalloc(b, &b->leaf_cnt, t)) || (rc = dalloc(b, &b->leaf_off, t)) ||
       (rc = dalloc(b, (char**)&b->scan_temp, scan_temp_bytes(0))))
     return fail(rc);
+  carve_build(t, b->lb_temp, &b->lb);
+  CM_HIP(hipMemset(b->lb_temp, 0, lbvh_build_temp_bytes(t)));
   CM_HIP(hipMemset(b->lb.parents, 0, sizeof(int) * m));
   CM_HIP(hipMemset(b->lb.amin, 0, sizeof(float4) * m));
   CM_HIP(hipMemset(b->lb.amax, 0, sizeof(float4) * m));

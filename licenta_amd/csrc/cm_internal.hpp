@@ -40,10 +40,6 @@ inline int nblocks(int64_t n, int block = kBlock) {
 }
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
-// warnings raised by kernels (same bits as CM_WARN_*), plus internal bits
-constexpr uint32_t kRefitOverflow = 1u << 16;  // refit incomplete: single-block fallback
-constexpr int kMaxClaims = 4;  // claimants kept per node for the refit (more -> fallback)
-
 // ---- primitives (k_prims.hip) --------------------------------------------
 size_t scan_temp_bytes(int n_max);
 // exclusive prefix sum of in[0, n) into out[0, n), n = d_n ? min(*d_n, n_max) : n_max;
@@ -61,14 +57,16 @@ struct LbvhBufs {
   float4 *amin = nullptr, *amax = nullptr;  // 2T-1 (internal nodes first)
   int *lefts = nullptr, *rights = nullptr, *parents = nullptr, *rmll = nullptr, *rmlr = nullptr;
   int *claim_cnt = nullptr, *claim_hi = nullptr;  // 2T-1: claimants, highest claimant
-  int* claim_list = nullptr;                        // kMaxClaims * (2T-1) claimant ids
-  int* visited = nullptr;                                              // T-1
-  int* done = nullptr;                                                 // T-1 (fallback)
-  int2* ranges = nullptr;                                              // T-1 key ranges
+  int2* ranges = nullptr;                           // T-1 key ranges
+  // refit: per node 1 = finished by its chunk, 0 = deferred (top_index = list slot)
+  int *state = nullptr, *top_list = nullptr, *top_index = nullptr, *ntop = nullptr;
+  float4 *tmin = nullptr, *tmax = nullptr;  // deferred-node scratch (large trees)
+  int *trm = nullptr, *tdone = nullptr;
   float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
   uint32_t* warn = nullptr;
 };
 size_t lbvh_build_temp_bytes(int t);
+size_t carve_build(int t, void* d_temp, LbvhBufs* b);  // b == nullptr: size only
 size_t lbvh_pairs_temp_bytes(int t, int cap);
 int launch_tri_update(int n, int t, const float4* x, const float4* p, const int4* tri, float th,
                       int bits, int normalize, float4* amin, float4* amax, uint64_t* keys,

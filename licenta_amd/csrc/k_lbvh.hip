@@ -4,25 +4,22 @@
 //                    (LicentaV2/LicentaV2/Physics/CudaDeformableUtils.cu:120-151)
 //  k_build        <- CudaBVH::_GenerateBVH2 / FindRange / FindSplit
 //                    (Collision/CudaBVH.cu:163-242), Delta/sgn (Core/CudaUtils.cu:338-362)
-//  k_refit        <- CudaBVH::ComputeTreeAABBs (Collision/CudaBVH.cu:128-149)
+//  k_refit_local,
+//  k_refit_top    <- CudaBVH::ComputeTreeAABBs (Collision/CudaBVH.cu:128-149)
 //  k_traverse     <- CudaBVH::_GetAABBCollisions (Collision/CudaBVH.cu:23-113)
 //
 // Layout: node AABBs are float4 min / float4 max arrays of 2T-1 entries, internal
 // nodes [0, T-1) then leaves [T-1, 2T-1), as in the reference. Parent writes are
-// made deterministic with claim records (count, highest claimant and the first
-// four claimant ids): parents[c] = highest claimant, which is what the
-// reference's last writer in thread order produces; the refit climbs through
-// every claimant so a node shared by several parents (midpoint split on tied
-// codes) is still the union of its two children.
+// made deterministic with claim records (count, highest claimant): parents[c] =
+// highest claimant, which is what the reference's last writer in thread order
+// produces. The refit pulls each node from its two children, so a node shared by
+// several parents (midpoint split on tied codes) is still the union of its two
+// children.
 #include "cm_internal.hpp"
 
 namespace cm {
 
 namespace {
-
-__device__ __forceinline__ int atomic_add_acq_rel(int* p, int v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // ---- per-triangle swept AABB and Morton key --------------------------------
 __device__ __forceinline__ void swept_aabb(const float4* __restrict__ x,
@@ -132,23 +129,19 @@ __device__ __forceinline__ int sgn(int a) { return signbit((float)a) ? -1 : 1; }
 
 __global__ void __launch_bounds__(kBlock) k_build_init(int t, int* __restrict__ cnt,
                                                        int* __restrict__ hi,
-                                                       int* __restrict__ visited,
-                                                       uint32_t* __restrict__ warn) {
+                                                       int* __restrict__ ntop) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 2 * t - 1) {
     cnt[i] = 0;
     hi[i] = -1;
   }
-  if (i < t - 1) visited[i] = 0;
-  // the refit-fallback request is per build; the CM_WARN_* bits stay sticky
-  if (i == 0) atomicAnd(warn, ~kRefitOverflow);
+  if (i == 0) *ntop = 0;  // deferred-node count of the refit
 }
 
 __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restrict__ k,
                                                   int midpoint, int* __restrict__ lefts,
                                                   int* __restrict__ rights, int* __restrict__ cnt,
-                                                  int* __restrict__ hi, int* __restrict__ list,
-                                                  int2* __restrict__ ranges,
+                                                  int* __restrict__ hi, int2* __restrict__ ranges,
                                                   uint32_t* __restrict__ warn) {
   const int id = blockIdx.x * kBlock + threadIdx.x;
   const int nin = t - 1;
@@ -187,23 +180,18 @@ __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restr
   lefts[id] = left;
   rights[id] = right;
   const int o1 = atomicAdd(&cnt[left], 1);
-  if (o1 < kMaxClaims) list[kMaxClaims * (int64_t)left + o1] = id;
   atomicMax(&hi[left], id);
   const int o2 = atomicAdd(&cnt[right], 1);
-  if (o2 < kMaxClaims) list[kMaxClaims * (int64_t)right + o2] = id;
   atomicMax(&hi[right], id);
   if (o1 > 0 || o2 > 0) atomicOr(warn, (uint32_t)CM_WARN_BROKEN_TREE);
 }
 
 __global__ void __launch_bounds__(kBlock) k_parents(int t, const int* __restrict__ cnt,
                                                     const int* __restrict__ hi,
-                                                    int* __restrict__ parents,
-                                                    uint32_t* __restrict__ warn) {
+                                                    int* __restrict__ parents) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= 2 * t - 1) return;
-  const int k = cnt[c];
-  if (k > 0) parents[c] = hi[c];
-  if (k > kMaxClaims) atomicOr(warn, kRefitOverflow);
+  if (cnt[c] > 0) parents[c] = hi[c];
 }
 
 // ---- bottom-up refit ------------------------------------------------------------
@@ -219,173 +207,224 @@ __device__ __forceinline__ void write_node(float4* __restrict__ nodes, int q, in
   o[3] = make_float4(rmn.z, rmx.x, rmx.y, rmx.z);
 }
 
-// node q = union of its two children (Collision/CudaBVH.cu:140-144), all in global memory
-__device__ __forceinline__ void union_global(int q, int nin, const int* __restrict__ lefts,
-                                             const int* __restrict__ rights,
-                                             float4* __restrict__ amin, float4* __restrict__ amax,
-                                             int* __restrict__ rmll, int* __restrict__ rmlr,
-                                             float4* __restrict__ nodes) {
-  const int l = lefts[q], r = rights[q];
-  const f3 lmn = xyz(amin[l]), lmx = xyz(amax[l]), rmn = xyz(amin[r]), rmx = xyz(amax[r]);
-  amin[q] = f4(vmin(lmn, rmn), 0.f);
-  amax[q] = f4(vmax(lmx, rmx), 0.f);
-  const int a = l >= nin ? l : max(rmll[l], rmlr[l]);
-  const int b = r >= nin ? r : max(rmll[r], rmlr[r]);
-  rmll[q] = a;
-  rmlr[q] = b;
-  write_node(nodes, q, l, r, a, b, lmn, lmx, rmn, rmx);
-}
+constexpr int kRefitCH = 1024;  // leaves (and node ids) per refit workgroup
+constexpr int kTopCap = 2048;   // deferred nodes the top pass keeps in LDS
 
-constexpr int kRefitCH = 1024;  // leaves per refit block
-constexpr int kClimbStack = 8;
-
-// One block per chunk of 1024 consecutive leaves. A node whose key range lies in
-// the chunk ("local") is finished in LDS with workgroup-scope arrival counters;
-// arrivals at nodes whose range crosses the chunk ("escapes", a few per chunk per
-// tree level) then climb with device-scope counters. Every claimant of a node is
-// notified, so a node shared by several parents (midpoint split on tied codes) is
-// still the union of its children; more than kMaxClaims claimants sets
-// kRefitOverflow and the single-block fallback recomputes the tree.
-__global__ void __launch_bounds__(kRefitCH) k_refit_chunk(
+// Refit, pass 1: one workgroup per chunk of 1024 consecutive leaves. Internal node
+// q belongs to the chunk holding index q (a Karras node's index lies in its key
+// range). Nodes whose range lies inside the chunk are finished here: in each
+// synchronised pass a node whose children are leaves of the chunk or nodes finished
+// in an earlier pass takes the union of its children (Collision/CudaBVH.cu:140-144),
+// all in LDS. Every other node of the chunk (its range crosses the chunk, or a child
+// is unfinished) is appended to the deferred list for k_refit_top. A node is the
+// union of its two children however many parents claimed it.
+__global__ void __launch_bounds__(kRefitCH) k_refit_local(
     int t, const int* __restrict__ lefts, const int* __restrict__ rights,
-    const int2* __restrict__ ranges, const int* __restrict__ cnt, const int* __restrict__ list,
-    int* __restrict__ visited, float4* __restrict__ amin,
-    float4* __restrict__ amax, int* __restrict__ rmll, int* __restrict__ rmlr,
-    float4* __restrict__ nodes, uint32_t* __restrict__ warn) {
-  __shared__ float4 smin[kRefitCH], smax[kRefitCH];
-  __shared__ int sarr[kRefitCH], srml[kRefitCH], srmr[kRefitCH];
-  __shared__ int2 esc[2 * kRefitCH];
-  __shared__ int esc_n;
+    const int2* __restrict__ ranges, float4* __restrict__ amin, float4* __restrict__ amax,
+    int* __restrict__ rmll, int* __restrict__ rmlr, float4* __restrict__ nodes,
+    int* __restrict__ state, int* __restrict__ top_list, int* __restrict__ top_index,
+    int* __restrict__ ntop) {
+  __shared__ float4 lmin[kRefitCH], lmax[kRefitCH];  // leaf boxes of the chunk
+  __shared__ float4 nmin[kRefitCH], nmax[kRefitCH];  // node boxes of the chunk
+  __shared__ int nrm[kRefitCH];                      // max(rmll, rmlr) of each node
+  __shared__ int ndone[kRefitCH];                    // pass that finished the node, 0 = not
   const int nin = t - 1, base = blockIdx.x * kRefitCH, tid = threadIdx.x;
-  sarr[tid] = 0;
-  if (tid == 0) esc_n = 0;
-  __syncthreads();
-  auto is_local = [&](int q) {
-    if (q >= nin) return false;
+  if (base + tid < t) {
+    lmin[tid] = amin[nin + base + tid];
+    lmax[tid] = amax[nin + base + tid];
+  }
+  const int q = base + tid;
+  int l = 0, r = 0;
+  bool pending = false;
+  if (q < nin) {
     const int2 rg = ranges[q];
-    return rg.x >= base && rg.y < base + kRefitCH;
-  };
-  uint32_t w = 0;
-  const int leaf = base + tid;
-  if (leaf < t) {
-    int stk[kClimbStack];
-    int top = 0;
-    stk[top++] = nin + leaf;
-    int guard = 0;
-    while (top > 0) {
-      if (++guard > 4 * kRefitCH) {
-        w |= kRefitOverflow;
-        break;
-      }
-      const int x = stk[--top];
-      const int k = cnt[x];
-      if (k > kMaxClaims) w |= kRefitOverflow;
-      const int kk = min(k, kMaxClaims);
-      for (int s = 0; s < kk; ++s) {
-        const int q = list[kMaxClaims * (int64_t)x + s];
-        if (is_local(q)) {
-          const int sq = q - base;
-          if (__hip_atomic_fetch_add(&sarr[sq], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 1) {
-            const int l = lefts[q], r = rights[q];
-            f3 lmn, lmx, rmn, rmx;
-            int a, b;
-            if (l >= nin) {
-              lmn = xyz(amin[l]); lmx = xyz(amax[l]); a = l;
-            } else {
-              lmn = xyz(smin[l - base]); lmx = xyz(smax[l - base]); a = max(srml[l - base], srmr[l - base]);
-            }
-            if (r >= nin) {
-              rmn = xyz(amin[r]); rmx = xyz(amax[r]); b = r;
-            } else {
-              rmn = xyz(smin[r - base]); rmx = xyz(smax[r - base]); b = max(srml[r - base], srmr[r - base]);
-            }
-            smin[sq] = f4(vmin(lmn, rmn), 0.f);
-            smax[sq] = f4(vmax(lmx, rmx), 0.f);
-            srml[sq] = a;
-            srmr[sq] = b;
-            write_node(nodes, q, l, r, a, b, lmn, lmx, rmn, rmx);
-            if (top < kClimbStack) stk[top++] = q; else w |= kRefitOverflow;
-          }
-        } else {
-          const int e = atomicAdd(&esc_n, 1);
-          if (e < 2 * kRefitCH) esc[e] = make_int2(x, q); else w |= kRefitOverflow;
-        }
-      }
+    if (rg.x >= base && rg.y < base + kRefitCH) {
+      l = lefts[q];
+      r = rights[q];
+      // children must be leaves or nodes of this chunk
+      const int lo = l >= nin ? l - nin : l, ro = r >= nin ? r - nin : r;
+      pending = lo >= base && lo < base + kRefitCH && ro >= base && ro < base + kRefitCH;
     }
   }
+  ndone[tid] = 0;
   __syncthreads();
-  {
-    const int q = base + tid;
-    if (q < nin && sarr[tid] >= 2 && is_local(q)) {
-      amin[q] = smin[tid];
-      amax[q] = smax[tid];
-      rmll[q] = srml[tid];
-      rmlr[q] = srmr[tid];
-    }
-  }
-  __threadfence();
-  __syncthreads();
-  const int ne = min(esc_n, 2 * kRefitCH);
-  for (int e = tid; e < ne; e += kRefitCH) {
-    int stk[kClimbStack];
-    int top = 0;
-    const int q0 = esc[e].y;
-    if (atomic_add_acq_rel(&visited[q0], 1) == 1) {
-      union_global(q0, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
-      stk[top++] = q0;
-    }
-    int guard = 0;
-    while (top > 0) {
-      if (++guard > 2 * t) {
-        w |= CM_WARN_REFIT_LIMIT;
-        break;
-      }
-      const int x = stk[--top];
-      const int k = cnt[x];
-      if (k > kMaxClaims) w |= kRefitOverflow;
-      const int kk = min(k, kMaxClaims);
-      for (int s = 0; s < kk; ++s) {
-        const int q = list[kMaxClaims * (int64_t)x + s];
-        if (atomic_add_acq_rel(&visited[q], 1) == 1) {
-          union_global(q, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
-          if (top < kClimbStack) stk[top++] = q; else w |= kRefitOverflow;
-        }
-      }
-    }
-  }
-  if (w) atomicOr(warn, w);
-}
-
-// Fallback when a node has more than two claimants or a climb stack filled:
-// one block recomputes every internal node, children first, in passes.
-__global__ void __launch_bounds__(1024) k_refit_fallback(int t, const int* __restrict__ lefts,
-                                                         const int* __restrict__ rights,
-                                                         int* __restrict__ done,
-                                                         float4* __restrict__ amin,
-                                                         float4* __restrict__ amax,
-                                                         int* __restrict__ rmll,
-                                                         int* __restrict__ rmlr,
-                                                         float4* __restrict__ nodes,
-                                                         const uint32_t* __restrict__ warn) {
-  if (!(*warn & kRefitOverflow)) return;
-  const int nin = t - 1;
-  for (int q = threadIdx.x; q < nin; q += blockDim.x) done[q] = 0;
-  __syncthreads();
-  for (int pass = 1; pass <= nin; ++pass) {
-    int changed = 0;
-    for (int q = threadIdx.x; q < nin; q += blockDim.x) {
-      if (done[q]) continue;
-      const int l = lefts[q], r = rights[q];
-      const bool rl = l >= nin || (done[l] != 0 && done[l] < pass);
-      const bool rr = r >= nin || (done[r] != 0 && done[r] < pass);
+  bool fin = false;
+  f3 lmn = mk3(0, 0, 0), lmx = lmn, rmn = lmn, rmx = lmn;
+  int a = 0, b = 0;
+  for (int pass = 1;; ++pass) {
+    bool changed = false;
+    if (pending) {
+      const bool rl = l >= nin || (ndone[l - base] != 0 && ndone[l - base] < pass);
+      const bool rr = r >= nin || (ndone[r - base] != 0 && ndone[r - base] < pass);
       if (rl && rr) {
-        union_global(q, nin, lefts, rights, amin, amax, rmll, rmlr, nodes);
-        done[q] = pass;
-        changed = 1;
+        if (l >= nin) {
+          lmn = xyz(lmin[l - nin - base]); lmx = xyz(lmax[l - nin - base]); a = l;
+        } else {
+          lmn = xyz(nmin[l - base]); lmx = xyz(nmax[l - base]); a = nrm[l - base];
+        }
+        if (r >= nin) {
+          rmn = xyz(lmin[r - nin - base]); rmx = xyz(lmax[r - nin - base]); b = r;
+        } else {
+          rmn = xyz(nmin[r - base]); rmx = xyz(nmax[r - base]); b = nrm[r - base];
+        }
+        nmin[tid] = f4(vmin(lmn, rmn), 0.f);
+        nmax[tid] = f4(vmax(lmx, rmx), 0.f);
+        nrm[tid] = max(a, b);
+        ndone[tid] = pass;
+        pending = false;
+        fin = true;
+        changed = true;
       }
     }
     if (!__syncthreads_or(changed)) break;
   }
+  if (q >= nin) return;
+  if (fin) {
+    amin[q] = nmin[tid];
+    amax[q] = nmax[tid];
+    rmll[q] = a;
+    rmlr[q] = b;
+    write_node(nodes, q, l, r, a, b, lmn, lmx, rmn, rmx);
+    state[q] = 1;
+  } else {
+    state[q] = 0;
+    const int j = atomicAdd(ntop, 1);
+    top_list[j] = q;
+    top_index[q] = j;
+  }
+}
+
+// child c of a deferred node: a leaf or a node finished by k_refit_local is final
+// in global memory (kind 0); otherwise it is deferred node top_index[c] (kind 1)
+__device__ __forceinline__ void top_child(int c, int nin, const int* __restrict__ state,
+                                          const int* __restrict__ top_index,
+                                          const float4* __restrict__ amin,
+                                          const float4* __restrict__ amax,
+                                          const int* __restrict__ rmll,
+                                          const int* __restrict__ rmlr, bool& deferred,
+                                          int& ref, f3& mn, f3& mx) {
+  deferred = c < nin && state[c] == 0;
+  if (deferred) {
+    ref = top_index[c];
+  } else {
+    mn = xyz(amin[c]);
+    mx = xyz(amax[c]);
+    ref = c >= nin ? c : max(rmll[c], rmlr[c]);
+  }
+}
+
+// Refit, pass 2: one workgroup finishes the deferred nodes with the same
+// synchronised passes. Up to kTopCap deferred nodes keep their boxes in LDS and
+// their children in registers (two per thread); beyond that the boxes live in
+// global scratch and node data is re-read each pass. Clears the list count.
+__global__ void __launch_bounds__(1024) k_refit_top(
+    int t, const int* __restrict__ lefts, const int* __restrict__ rights,
+    float4* __restrict__ amin, float4* __restrict__ amax, int* __restrict__ rmll,
+    int* __restrict__ rmlr, float4* __restrict__ nodes, const int* __restrict__ state,
+    const int* __restrict__ top_list, const int* __restrict__ top_index, int* __restrict__ ntop,
+    float4* __restrict__ gmin, float4* __restrict__ gmax, int* __restrict__ grm,
+    int* __restrict__ gdone, uint32_t* __restrict__ warn) {
+  __shared__ float4 smin[kTopCap], smax[kTopCap];
+  __shared__ int srm[kTopCap], sdone[kTopCap];
+  const int nin = t - 1, tid = threadIdx.x;
+  const int nt = *ntop;
+  bool left_over = false;
+  if (nt <= kTopCap) {
+    constexpr int kPer = kTopCap / 1024;
+    int q[kPer], l[kPer], r[kPer], la[kPer], ra[kPer];
+    bool own[kPer], ltop[kPer], rtop[kPer];
+    f3 lmn[kPer], lmx[kPer], rmn[kPer], rmx[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int j = tid + k * 1024;
+      own[k] = j < nt;
+      if (own[k]) {
+        q[k] = top_list[j];
+        l[k] = lefts[q[k]];
+        r[k] = rights[q[k]];
+        top_child(l[k], nin, state, top_index, amin, amax, rmll, rmlr, ltop[k], la[k], lmn[k],
+                  lmx[k]);
+        top_child(r[k], nin, state, top_index, amin, amax, rmll, rmlr, rtop[k], ra[k], rmn[k],
+                  rmx[k]);
+        sdone[j] = 0;
+      }
+    }
+    __syncthreads();
+    for (int pass = 1;; ++pass) {
+      bool changed = false;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int j = tid + k * 1024;
+        if (!own[k] || sdone[j] != 0) continue;
+        const bool rl = !ltop[k] || (sdone[la[k]] != 0 && sdone[la[k]] < pass);
+        const bool rr = !rtop[k] || (sdone[ra[k]] != 0 && sdone[ra[k]] < pass);
+        if (!(rl && rr)) continue;
+        if (ltop[k]) {
+          const int c = la[k];
+          lmn[k] = xyz(smin[c]); lmx[k] = xyz(smax[c]); la[k] = srm[c];
+        }
+        if (rtop[k]) {
+          const int c = ra[k];
+          rmn[k] = xyz(smin[c]); rmx[k] = xyz(smax[c]); ra[k] = srm[c];
+        }
+        smin[j] = f4(vmin(lmn[k], rmn[k]), 0.f);
+        smax[j] = f4(vmax(lmx[k], rmx[k]), 0.f);
+        srm[j] = max(la[k], ra[k]);
+        sdone[j] = pass;
+        changed = true;
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int j = tid + k * 1024;
+      if (!own[k]) continue;
+      if (sdone[j] == 0) {
+        left_over = true;
+        continue;
+      }
+      amin[q[k]] = smin[j];
+      amax[q[k]] = smax[j];
+      rmll[q[k]] = la[k];
+      rmlr[q[k]] = ra[k];
+      write_node(nodes, q[k], l[k], r[k], la[k], ra[k], lmn[k], lmx[k], rmn[k], rmx[k]);
+    }
+  } else {
+    for (int j = tid; j < nt; j += 1024) gdone[j] = 0;
+    __syncthreads();
+    for (int pass = 1;; ++pass) {
+      bool changed = false;
+      for (int j = tid; j < nt; j += 1024) {
+        if (gdone[j] != 0) continue;
+        const int qq = top_list[j], ll = lefts[qq], rr = rights[qq];
+        bool lt, rt;
+        int a, b;
+        f3 lmn, lmx, rmn, rmx;
+        top_child(ll, nin, state, top_index, amin, amax, rmll, rmlr, lt, a, lmn, lmx);
+        top_child(rr, nin, state, top_index, amin, amax, rmll, rmlr, rt, b, rmn, rmx);
+        if (lt && !(gdone[a] != 0 && gdone[a] < pass)) continue;
+        if (rt && !(gdone[b] != 0 && gdone[b] < pass)) continue;
+        if (lt) { lmn = xyz(gmin[a]); lmx = xyz(gmax[a]); a = grm[a]; }
+        if (rt) { rmn = xyz(gmin[b]); rmx = xyz(gmax[b]); b = grm[b]; }
+        gmin[j] = f4(vmin(lmn, rmn), 0.f);
+        gmax[j] = f4(vmax(lmx, rmx), 0.f);
+        grm[j] = max(a, b);
+        amin[qq] = gmin[j];
+        amax[qq] = gmax[j];
+        rmll[qq] = a;
+        rmlr[qq] = b;
+        write_node(nodes, qq, ll, rr, a, b, lmn, lmx, rmn, rmx);
+        __threadfence_block();
+        gdone[j] = pass;
+        changed = true;
+      }
+      if (!__syncthreads_or(changed)) break;
+    }
+    for (int j = tid; j < nt; j += 1024)
+      if (gdone[j] == 0) left_over = true;
+  }
+  if (__syncthreads_or(left_over) && tid == 0) atomicOr(warn, (uint32_t)CM_WARN_REFIT_LIMIT);
+  if (tid == 0) *ntop = 0;
 }
 
 // ---- traversal ----------------------------------------------------------------
@@ -537,11 +576,11 @@ int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStrea
   const int t = b.t;
   if (t < 2) return CM_OK;
   hipLaunchKernelGGL(k_build_init, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
-                     b.claim_hi, b.visited, b.warn);
+                     b.claim_hi, b.ntop);
   hipLaunchKernelGGL(k_build, dim3(nblocks(t - 1)), dim3(kBlock), 0, s, t, keys, midpoint, b.lefts,
-                     b.rights, b.claim_cnt, b.claim_hi, b.claim_list, b.ranges, b.warn);
+                     b.rights, b.claim_cnt, b.claim_hi, b.ranges, b.warn);
   hipLaunchKernelGGL(k_parents, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
-                     b.claim_hi, b.parents, b.warn);
+                     b.claim_hi, b.parents);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -549,11 +588,12 @@ int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStrea
 int launch_refit(const LbvhBufs& b, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return CM_OK;
-  hipLaunchKernelGGL(k_refit_chunk, dim3((t + kRefitCH - 1) / kRefitCH), dim3(kRefitCH), 0, s, t,
-                     b.lefts, b.rights, b.ranges, b.claim_cnt, b.claim_list, b.visited,
-                     b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.warn);
-  hipLaunchKernelGGL(k_refit_fallback, dim3(1), dim3(1024), 0, s, t, b.lefts, b.rights, b.done,
-                     b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.warn);
+  hipLaunchKernelGGL(k_refit_local, dim3((t + kRefitCH - 1) / kRefitCH), dim3(kRefitCH), 0, s, t,
+                     b.lefts, b.rights, b.ranges, b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.state,
+                     b.top_list, b.top_index, b.ntop);
+  hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, t, b.lefts, b.rights, b.amin, b.amax,
+                     b.rmll, b.rmlr, b.nodes, b.state, b.top_list, b.top_index, b.ntop, b.tmin,
+                     b.tmax, b.trm, b.tdone, b.warn);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -602,13 +642,34 @@ int launch_copy_i4(int64_t n, const int4* src, int4* dst, hipStream_t s) {
   return CM_OK;
 }
 
-size_t lbvh_build_temp_bytes(int t) {
+// Build/refit scratch carved from one buffer; with b == nullptr only the size.
+size_t carve_build(int t, void* d_temp, LbvhBufs* b) {
   const size_t m = t > 0 ? 2 * (size_t)t - 1 : 0;
   const size_t tt = (size_t)(t > 0 ? t : 1);
-  return 2 * align_up(sizeof(int) * m) + align_up(sizeof(int) * kMaxClaims * m) +
-         2 * align_up(sizeof(int) * tt) +
-         align_up(sizeof(int2) * tt) + align_up(4 * sizeof(float4) * tt);
+  char* tp = (char*)d_temp;
+  size_t off = 0;
+  auto take = [&](auto** dst, size_t bytes) {
+    if (b) *dst = (std::remove_reference_t<decltype(**dst)>*)(tp + off);
+    off += align_up(bytes);
+  };
+  LbvhBufs dummy;
+  LbvhBufs* o = b ? b : &dummy;
+  take(&o->claim_cnt, sizeof(int) * m);
+  take(&o->claim_hi, sizeof(int) * m);
+  take(&o->state, sizeof(int) * tt);
+  take(&o->top_list, sizeof(int) * tt);
+  take(&o->top_index, sizeof(int) * tt);
+  take(&o->ntop, sizeof(int));
+  take(&o->ranges, sizeof(int2) * tt);
+  take(&o->nodes, 4 * sizeof(float4) * tt);
+  take(&o->tmin, sizeof(float4) * tt);
+  take(&o->tmax, sizeof(float4) * tt);
+  take(&o->trm, sizeof(int) * tt);
+  take(&o->tdone, sizeof(int) * tt);
+  return off;
 }
+
+size_t lbvh_build_temp_bytes(int t) { return carve_build(t, nullptr, nullptr); }
 size_t lbvh_pairs_temp_bytes(int t, int cap) {
   return align_up(sizeof(int2) * (size_t)t * cap) + 2 * align_up(sizeof(int) * (size_t)t) +
          align_up(4 * sizeof(float4) * (size_t)t) + scan_temp_bytes(t);
@@ -664,23 +725,6 @@ extern "C" int cm_lbvh_sort(int32_t t, int32_t key_bits, uint64_t* d_keys, uint3
 
 extern "C" size_t cm_lbvh_build_temp_bytes(int32_t t) { return lbvh_build_temp_bytes(t); }
 
-static void carve_build(int t, void* d_temp, LbvhBufs& b) {
-  const size_t m = 2 * (size_t)t - 1;
-  char* tp = (char*)d_temp;
-  b.claim_cnt = (int*)tp;
-  tp += align_up(sizeof(int) * m);
-  b.claim_hi = (int*)tp;
-  tp += align_up(sizeof(int) * m);
-  b.claim_list = (int*)tp;
-  tp += align_up(sizeof(int) * kMaxClaims * m);
-  b.visited = (int*)tp;
-  tp += align_up(sizeof(int) * (size_t)t);
-  b.done = (int*)tp;
-  tp += align_up(sizeof(int) * (size_t)t);
-  b.ranges = (int2*)tp;
-  tp += align_up(sizeof(int2) * (size_t)t);
-  b.nodes = (float4*)tp;
-}
 
 extern "C" int cm_lbvh_build(int32_t t, const uint64_t* d_keys, int32_t midpoint_split,
                              int32_t* d_lefts, int32_t* d_rights, int32_t* d_parents,
@@ -696,7 +740,7 @@ extern "C" int cm_lbvh_build(int32_t t, const uint64_t* d_keys, int32_t midpoint
   b.rights = d_rights;
   b.parents = d_parents;
   b.warn = d_warn;
-  carve_build(t, d_temp, b);
+  carve_build(t, d_temp, &b);
   return launch_build(b, d_keys, midpoint_split, (hipStream_t)stream);
 }
 
@@ -720,7 +764,7 @@ extern "C" int cm_lbvh_refit(int32_t t, const int32_t* d_lefts, const int32_t* d
   b.rmll = d_rmll;
   b.rmlr = d_rmlr;
   b.warn = d_warn;
-  carve_build(t, d_temp, b);
+  carve_build(t, d_temp, &b);
   return launch_refit(b, (hipStream_t)stream);
 }
 
