@@ -94,7 +94,7 @@ extern "C" {
 #define CM_RC_MIDPOINT_SPLIT (1u << 2)
 #define CM_RC_ALL (CM_RC_STALE_LEAF_AABB | CM_RC_BEND_ABS | CM_RC_MIDPOINT_SPLIT)
 
-/* POD replacement for Physics::ClothParams (Physics/Structs.h:230-268). */
+/* POD replacement for Physics::ClothParams (Physics/Structs.h:23-61). */
 typedef struct cm_params {
   float timestep;          /* substep dt in seconds                              */
   float gravity[3];        /* m/s^2                                              */
@@ -120,7 +120,7 @@ typedef struct cm_params {
   int32_t reserved[7];
 } cm_params;
 
-/* Same fields as Physics::PrimitiveContact (Physics/Structs.h:313-336), 48 B. */
+/* Same fields as Physics::PrimitiveContact (Physics/Structs.h:106-129), 48 B. */
 typedef struct cm_contact {
   int32_t v[4];
   float w[4];

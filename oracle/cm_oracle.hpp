@@ -135,7 +135,7 @@ inline bool aabb_overlap(V3<float> amin, V3<float> amax, V3<float> bmin, V3<floa
 }
 
 struct Tri {
-  int32_t v1, v2, v3, feat;  // Physics/Structs.h:383-395 (m_v1..m_v3, m_assignedFeatures)
+  int32_t v1, v2, v3, feat;  // Physics/Structs.h:176-188 (m_v1..m_v3, m_assignedFeatures)
 };
 
 // ---------------------------------------------------------------------------
@@ -576,7 +576,7 @@ inline bool ccd_ee(cm_contact& c, const F3* x, const F3* v, float th, float dt) 
   return false;
 }
 
-inline cm_contact make_contact(int a, int b, int c, int d) {  // Physics/Structs.h:329-335
+inline cm_contact make_contact(int a, int b, int c, int d) {  // Physics/Structs.h:112-128
   cm_contact r;
   r.v[0] = a;
   r.v[1] = b;
