@@ -47,6 +47,10 @@ struct cm_body {
   float2* b_kpphi0 = nullptr;
   int* vprefix = nullptr;
   uint32_t* vslot = nullptr;
+  int ntiles = 0;
+  int* tstart = nullptr;
+  int* eprefix = nullptr;
+  TileEntry* entries = nullptr;
   int* lra_fixed = nullptr;
   float* lra_l0 = nullptr;
   uint8_t* lra_flag = nullptr;
@@ -161,7 +165,65 @@ struct HostTopo {
   std::vector<int4> tri;
   std::vector<int> lra_fixed;
   std::vector<float> lra_l0;
+  std::vector<int> tstart, eprefix;  // Jacobi tiles (empty: slot-parallel solve)
+  std::vector<TileEntry> entries;
 };
+
+// Cut the vertices into tiles of at most kTileV consecutive vertices and kTileSlots
+// slots, and list for each tile every constraint that touches it once, with the
+// tile-local slot of each of its roles. Leaves the tiles empty when one vertex has
+// more slots than a tile holds.
+void build_tiles(int n, HostTopo& h) {
+  const std::vector<int>& pre = h.prefix;
+  std::vector<int> ts{0};
+  for (int v = 0; v < n;) {
+    const int start = v;
+    while (v < n && v - start < kTileV && pre[v + 1] - pre[start] <= kTileSlots) ++v;
+    if (v == start) return;  // a vertex with more than kTileSlots slots
+    ts.push_back(v);
+  }
+  const int nt = (int)ts.size() - 1;
+  std::vector<int> seen_s(h.sid.size(), -1), seen_b(h.bid.size(), -1);  // entry of this tile
+  std::vector<int> ep{0};
+  std::vector<TileEntry> ent;
+  for (int tile = 0; tile < nt; ++tile) {
+    const int s0 = pre[ts[tile]], s1 = pre[ts[tile + 1]];
+    const int base = (int)ent.size();
+    for (int k = s0; k < s1; ++k) {
+      const uint32_t w = h.vslot[k];
+      const int idx = (int)(w & 0x1fffffffu), role = (int)((w >> 29) & 3u);
+      const bool bend = (w & 0x80000000u) != 0;
+      int& seen = bend ? seen_b[idx] : seen_s[idx];
+      if (seen < base) {  // first slot of this constraint in this tile
+        seen = (int)ent.size();
+        TileEntry e;
+        if (bend) {
+          e.ids = h.bid[idx];
+          e.a = h.b_kpphi0[idx].x;
+          e.b = h.b_kpphi0[idx].y;
+        } else {
+          e.ids = make_int4(h.sid[idx].x, h.sid[idx].y, -1, -1);
+          e.a = h.s_l0kp[idx].x;
+          e.b = h.s_l0kp[idx].y;
+        }
+        e.s01 = e.s23 = 0xffffffffu;
+        ent.push_back(e);
+      }
+      TileEntry& e = ent[seen];
+      const uint32_t off = (uint32_t)(k - s0);
+      uint32_t& word = role < 2 ? e.s01 : e.s23;
+      const int sh = (role & 1) * 16;
+      word = (word & ~(0xffffu << sh)) | (off << sh);
+    }
+    // distance constraints first, then bends: a wave then mostly runs one path
+    std::stable_partition(ent.begin() + base, ent.end(),
+                          [](const TileEntry& e) { return e.ids.z < 0; });
+    ep.push_back((int)ent.size());
+  }
+  h.tstart = std::move(ts);
+  h.eprefix = std::move(ep);
+  h.entries = std::move(ent);
+}
 
 int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* tri, int t,
                    const uint8_t* pinned, const int32_t* dist_pairs, int n_dist,
@@ -267,6 +329,7 @@ int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* 
     for (int j = 0; j < F; ++j)
       if (!pinned[i])
         h.lra_l0[(size_t)i * F + j] = host_dist(xyz, i, h.lra_fixed[j]) * (1.f + P.strain_limit);
+  build_tiles(n, h);
   return CM_OK;
 }
 
@@ -287,6 +350,10 @@ PbdBufs pbd_bufs(cm_body* b) {
   pb.b_kpphi0 = b->b_kpphi0;
   pb.vprefix = b->vprefix;
   pb.vslot = b->vslot;
+  pb.ntiles = b->ntiles;
+  pb.tstart = b->tstart;
+  pb.eprefix = b->eprefix;
+  pb.entries = b->entries;
   pb.lra_fixed = b->lra_fixed;
   pb.lra_l0 = b->lra_l0;
   pb.lra_flag = b->lra_flag;
@@ -495,6 +562,7 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   b->S = (int)h.sid.size();
   b->B = (int)h.bid.size();
   b->F = (int)h.lra_fixed.size();
+  b->ntiles = h.tstart.empty() ? 0 : (int)h.tstart.size() - 1;
   b->nslots = h.prefix[n];
   for (auto& e : h.sid) {
     b->h_sid.push_back(e.x);
@@ -521,6 +589,9 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = upload(b, &b->s_l0kp, h.s_l0kp)) || (rc = upload(b, &b->bid, h.bid)) ||
       (rc = upload(b, &b->b_kpphi0, h.b_kpphi0)) || (rc = upload(b, &b->vprefix, h.prefix)) ||
       (rc = upload(b, &b->vslot, h.vslot)) || (rc = upload(b, &b->lra_fixed, h.lra_fixed)) ||
+      (!h.tstart.empty() && ((rc = upload(b, &b->tstart, h.tstart)) ||
+                             (rc = upload(b, &b->eprefix, h.eprefix)) ||
+                             (rc = upload(b, &b->entries, h.entries)))) ||
       (rc = upload(b, &b->lra_l0, h.lra_l0)) ||
       (rc = dalloc(b, &b->lra_flag, (size_t)n * std::max(b->F, 1))) ||
       (rc = dalloc(b, &b->ground, n)) ||

@@ -112,6 +112,17 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s);
 
 // ---- PBD (k_pbd.hip) -------------------------------------------------------
+// Vertex tiles of the Jacobi kernel: at most kTileV consecutive vertices holding at
+// most kTileSlots constraint slots. Each tile lists the constraints touching it once.
+constexpr int kTileV = 64;
+constexpr int kTileSlots = 3072;
+struct alignas(16) TileEntry {
+  int4 ids;            // vertex ids; ids.z < 0 marks a distance constraint (ids.x, ids.y)
+  float a, b;          // distance: l0, kp; bend: kp, phi0
+  uint32_t s01, s23;   // tile-local slot of roles 0..3, 16 bits each, 0xffff = not in tile
+};
+static_assert(sizeof(TileEntry) == 32, "TileEntry is two 16-byte loads");
+
 struct PbdBufs {
   int n = 0, S = 0, B = 0, F = 0;
   float4 *x = nullptr, *p = nullptr, *p2 = nullptr, *v = nullptr;  // .w of x/p = inverse mass
@@ -122,6 +133,10 @@ struct PbdBufs {
   const float2* b_kpphi0 = nullptr;
   const int* vprefix = nullptr;
   const uint32_t* vslot = nullptr;
+  int ntiles = 0;  // 0: slot-parallel solve (a vertex has more than kTileSlots slots)
+  const int* tstart = nullptr;
+  const int* eprefix = nullptr;
+  const TileEntry* entries = nullptr;
   const int* lra_fixed = nullptr;
   const float* lra_l0 = nullptr;
   uint8_t* lra_flag = nullptr;

@@ -253,6 +253,14 @@ template <> __device__ __forceinline__ float tacos<float>(float v) { return acos
 template <> __device__ __forceinline__ double tacos<double>(double v) { return acos(v); }
 template <class T> __device__ __forceinline__ T tlen(tv<T> a) { return tsqrt<T>(a.x * a.x + a.y * a.y + a.z * a.z); }
 template <class T> __device__ __forceinline__ tv<T> tfrom(float4 a) { return {(T)a.x, (T)a.y, (T)a.z}; }
+// v / s per component for the bend: FP32 divides like the reference; the FP64 form
+// (CM_F_BEND_FP64, this project's own precision choice) multiplies by one
+// reciprocal, the same operations as the oracle's FP64 bend.
+template <class T> __device__ __forceinline__ tv<T> bdiv(tv<T> v, T s);
+template <> __device__ __forceinline__ tv<float> bdiv<float>(tv<float> v, float s) { return tdiv(v, s); }
+template <> __device__ __forceinline__ tv<double> bdiv<double>(tv<double> v, double s) {
+  return tmul(v, 1.0 / s);
+}
 template <class T> __device__ __forceinline__ f3 tto(tv<T> a) { return mk3((float)a.x, (float)a.y, (float)a.z); }
 
 // Physics/CudaPBD.cu:1024-1147 for one vertex role, evaluated in T
@@ -273,7 +281,7 @@ __device__ f3 bend_corr(int4 ids, float2 kpphi0, int role, int abs_mode,
   const T l23 = tlen(c23), l24 = tlen(c24);
   const T eps = (T)1e-8f;
   if (l23 <= eps || l24 <= eps) return mk3(0, 0, 0);
-  const tv<T> n1 = tdiv(c23, l23), n2 = tdiv(c24, l24);
+  const tv<T> n1 = bdiv<T>(c23, l23), n2 = bdiv<T>(c24, l24);
   const T d = fmin(fmax((T)-1, tdot(n1, n2)), (T)1);
   T phi = tacos<T>(d);
   const T PIf = (T)3.14159265359f;
@@ -290,10 +298,10 @@ __device__ f3 bend_corr(int4 ids, float2 kpphi0, int role, int abs_mode,
     phi = (T)0;
     if (phi == phi0) return mk3(0, 0, 0);
   }
-  const tv<T> q3 = tdiv(tadd(tcross(p2, n2), tmul(tcross(n1, p2), d)), l23);
-  const tv<T> q4 = tdiv(tadd(tcross(p2, n1), tmul(tcross(n2, p2), d)), l24);
-  const tv<T> q2 = tsub(tneg(tdiv(tadd(tcross(p3, n2), tmul(tcross(n1, p3), d)), l23)),
-                        tdiv(tadd(tcross(p4, n1), tmul(tcross(n2, p4), d)), l24));
+  const tv<T> q3 = bdiv<T>(tadd(tcross(p2, n2), tmul(tcross(n1, p2), d)), l23);
+  const tv<T> q4 = bdiv<T>(tadd(tcross(p2, n1), tmul(tcross(n2, p2), d)), l24);
+  const tv<T> q2 = tsub(tneg(bdiv<T>(tadd(tcross(p3, n2), tmul(tcross(n1, p3), d)), l23)),
+                        bdiv<T>(tadd(tcross(p4, n1), tmul(tcross(n2, p4), d)), l24));
   const tv<T> q1 = tsub(tsub(tneg(q2), q3), q4);
   const T term = tsqrt<T>((T)1 - d * d) * (phi - phi0) * kp;
   const T s1 = tdot(q1, q1), s2 = tdot(q2, q2), s3 = tdot(q3, q3), s4 = tdot(q4, q4);
@@ -306,6 +314,72 @@ __device__ f3 bend_corr(int4 ids, float2 kpphi0, int role, int abs_mode,
     case 2: return tto(tmul(q3, -(w3 * ts)));
     default: return tto(tmul(q4, -(w4 * ts)));
   }
+}
+
+// stretch_corr for both roles at once (same operations as stretch_corr)
+__device__ __forceinline__ void stretch_both(int2 ids, float2 l0kp, const float4* __restrict__ p,
+                                             f3& c0, f3& c1) {
+  const float4 a = p[ids.x], b = p[ids.y];
+  c0 = c1 = mk3(0, 0, 0);
+  const float w1 = a.w, w2 = b.w, ws = w1 + w2;
+  if (ws == 0) return;
+  const f3 diff = sub(xyz(a), xyz(b));
+  const float l = len(diff);
+  if (l <= 1e-8f) return;
+  const f3 nn = divs(diff, l);
+  const f3 tt = divs(mul(mul(nn, l - l0kp.x), l0kp.y), ws);
+  c0 = mul(tt, -w1);
+  c1 = mul(tt, w2);
+}
+
+// bend_corr for all four roles at once (same operations as bend_corr)
+template <class T>
+__device__ __forceinline__ void bend_all(int4 ids, float2 kpphi0, int abs_mode,
+                                         const float4* __restrict__ p, f3 c[4]) {
+  c[0] = c[1] = c[2] = c[3] = mk3(0, 0, 0);
+  const float4 a1 = p[ids.x], a2 = p[ids.y], a3 = p[ids.z], a4 = p[ids.w];
+  const T w1 = (T)a1.w, w2 = (T)a2.w, w3 = (T)a3.w, w4 = (T)a4.w;
+  tv<T> p2 = tfrom<T>(a2), p3 = tfrom<T>(a3), p4 = tfrom<T>(a4);
+  if (!abs_mode) {
+    const tv<T> p1 = tfrom<T>(a1);
+    p2 = tsub(p2, p1);
+    p3 = tsub(p3, p1);
+    p4 = tsub(p4, p1);
+  }
+  const T kp = (T)kpphi0.x, phi0 = (T)kpphi0.y;
+  const tv<T> c23 = tcross(p2, p3), c24 = tcross(p2, p4);
+  const T l23 = tlen(c23), l24 = tlen(c24);
+  const T eps = (T)1e-8f;
+  if (l23 <= eps || l24 <= eps) return;
+  const tv<T> n1 = bdiv<T>(c23, l23), n2 = bdiv<T>(c24, l24);
+  const T d = fmin(fmax((T)-1, tdot(n1, n2)), (T)1);
+  T phi = tacos<T>(d);
+  const T PIf = (T)3.14159265359f;
+  if (d == (T)-1) {
+    phi = PIf;
+    if (phi == phi0 || !abs_mode) return;
+    c[0] = tto(tdiv(n1, (T)100));
+    c[1] = tto(tdiv(n2, (T)100));
+    return;
+  }
+  if (d == (T)1) {
+    phi = (T)0;
+    if (phi == phi0) return;
+  }
+  const tv<T> q3 = bdiv<T>(tadd(tcross(p2, n2), tmul(tcross(n1, p2), d)), l23);
+  const tv<T> q4 = bdiv<T>(tadd(tcross(p2, n1), tmul(tcross(n2, p2), d)), l24);
+  const tv<T> q2 = tsub(tneg(bdiv<T>(tadd(tcross(p3, n2), tmul(tcross(n1, p3), d)), l23)),
+                        bdiv<T>(tadd(tcross(p4, n1), tmul(tcross(n2, p4), d)), l24));
+  const tv<T> q1 = tsub(tsub(tneg(q2), q3), q4);
+  const T term = tsqrt<T>((T)1 - d * d) * (phi - phi0) * kp;
+  const T s1 = tdot(q1, q1), s2 = tdot(q2, q2), s3 = tdot(q3, q3), s4 = tdot(q4, q4);
+  const T sf = w1 * s1 + w2 * s2 + w3 * s3 + w4 * s4;
+  if (sf <= eps) return;
+  const T ts = term / sf;
+  c[0] = tto(tmul(q1, -(w1 * ts)));
+  c[1] = tto(tmul(q2, -(w2 * ts)));
+  c[2] = tto(tmul(q3, -(w3 * ts)));
+  c[3] = tto(tmul(q4, -(w4 * ts)));
 }
 
 __device__ __forceinline__ f3 vf_corr(int4 v, float4 w, int role, float th,
@@ -372,6 +446,10 @@ struct SolveArgs {
   const int* d_nvf;
   const int4 *vf_v, *ee_v;
   const float4 *vf_w, *ee_w;
+  // vertex tiles: tstart[ntiles + 1] vertex ranges, eprefix[ntiles + 1] entry ranges
+  const int* tstart;
+  const int* eprefix;
+  const TileEntry* entries;
 };
 
 // one static slot's correction: slot word = bend flag (bit 31), role (bits 29-30),
@@ -394,21 +472,37 @@ __device__ __forceinline__ f3 rec_corr(const SolveArgs& a, uint32_t r, int nvf,
                   : ee_corr(a.ee_v[ci - nvf], a.ee_w[ci - nvf], role, a.th, pin);
 }
 
-constexpr int kSolveV = 64;        // vertices per workgroup
-constexpr int kSolveChunk = 2048;  // slot corrections staged in LDS per pass
+constexpr int kSolveV = 64;  // vertices per workgroup (slot-parallel form)
+constexpr int kLds = kTileSlots;  // slot corrections staged in LDS per pass
 
-// One Jacobi iteration over kSolveV consecutive vertices. The workgroup's slots
-// (and then its dynamic records) are evaluated in parallel, kSolveChunk at a time,
-// into LDS; each vertex then adds its own entries in slot order, so every per-vertex
-// sum has the reference's order and rounding.
+__device__ __forceinline__ void lds_put(float* cx, float* cy, float* cz, uint32_t off, f3 c) {
+  cx[off] = c.x;
+  cy[off] = c.y;
+  cz[off] = c.z;
+}
+
+// One Jacobi iteration over a range of consecutive vertices. The range's static
+// corrections are staged in LDS: in the tiled form (kTiles) each constraint that
+// touches the tile is evaluated once, from the tile's entry list, and its
+// corrections go to the slots of the tile's vertices; in the slot-parallel form
+// every slot re-evaluates its constraint, kLds slots per pass. The range's dynamic
+// records follow the same way. Each vertex then adds its own entries in slot
+// order, so every per-vertex sum has the reference's order and rounding.
+template <bool kTiles>
 __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __restrict__ pin,
                                                   float4* __restrict__ pout) {
-  __shared__ float cx[kSolveChunk], cy[kSolveChunk], cz[kSolveChunk];
-  const int v0 = blockIdx.x * kSolveV;
-  const int v1 = min(v0 + kSolveV, a.n);
+  __shared__ float cx[kLds], cy[kLds], cz[kLds];
+  int v0, v1;
+  if (kTiles) {
+    v0 = a.tstart[blockIdx.x];
+    v1 = a.tstart[blockIdx.x + 1];
+  } else {
+    v0 = blockIdx.x * kSolveV;
+    v1 = min(v0 + kSolveV, a.n);
+  }
   const int tid = threadIdx.x;
   const int i = v0 + tid;
-  const bool mine = tid < kSolveV && i < v1;
+  const bool mine = i < v1;  // tiles hold at most kBlock vertices
   float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
   f3 total = mk3(0, 0, 0);
   int cnt = 0;
@@ -447,13 +541,32 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
     int k = mine ? a.vprefix[i] : 0;
     const int e = mine ? a.vprefix[i + 1] : 0;
     f3 acc = mk3(0, 0, 0);
-    for (int c0 = s0; c0 < s1; c0 += kSolveChunk) {
-      const int c1 = min(c0 + kSolveChunk, s1);
-      for (int q = c0 + tid; q < c1; q += kBlock) {
-        const f3 c = slot_corr(a, a.vslot[q], pin);
-        cx[q - c0] = c.x;
-        cy[q - c0] = c.y;
-        cz[q - c0] = c.z;
+    for (int c0 = s0; c0 < s1; c0 += kLds) {
+      const int c1 = min(c0 + kLds, s1);
+      if (kTiles) {  // a tile's slots fit one pass (c0 == s0)
+        const int eb = a.eprefix[blockIdx.x], ee = a.eprefix[blockIdx.x + 1];
+        for (int q = eb + tid; q < ee; q += kBlock) {
+          const TileEntry te = a.entries[q];
+          if (te.ids.z < 0) {
+            f3 c0v, c1v;
+            stretch_both(make_int2(te.ids.x, te.ids.y), make_float2(te.a, te.b), pin, c0v, c1v);
+            if ((te.s01 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, te.s01 & 0xffffu, c0v);
+            if ((te.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, te.s01 >> 16, c1v);
+          } else {
+            f3 cr[4];
+            if (a.bend_fp64)
+              bend_all<double>(te.ids, make_float2(te.a, te.b), a.abs_mode, pin, cr);
+            else
+              bend_all<float>(te.ids, make_float2(te.a, te.b), a.abs_mode, pin, cr);
+            if ((te.s01 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, te.s01 & 0xffffu, cr[0]);
+            if ((te.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, te.s01 >> 16, cr[1]);
+            if ((te.s23 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, te.s23 & 0xffffu, cr[2]);
+            if ((te.s23 >> 16) != 0xffffu) lds_put(cx, cy, cz, te.s23 >> 16, cr[3]);
+          }
+        }
+      } else {
+        for (int q = c0 + tid; q < c1; q += kBlock)
+          lds_put(cx, cy, cz, q - c0, slot_corr(a, a.vslot[q], pin));
       }
       __syncthreads();
       const int kend = min(e, c1);
@@ -482,14 +595,10 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
       const bool any = mine && e > k;
       f3 acc = mk3(0, 0, 0);
       int c = 0;
-      for (int c0 = r0; c0 < r1; c0 += kSolveChunk) {
-        const int c1 = min(c0 + kSolveChunk, r1);
-        for (int q = c0 + tid; q < c1; q += kBlock) {
-          const f3 val = rec_corr(a, a.drec[q], nvf, pin);
-          cx[q - c0] = val.x;
-          cy[q - c0] = val.y;
-          cz[q - c0] = val.z;
-        }
+      for (int c0 = r0; c0 < r1; c0 += kLds) {
+        const int c1 = min(c0 + kLds, r1);
+        for (int q = c0 + tid; q < c1; q += kBlock)
+          lds_put(cx, cy, cz, q - c0, rec_corr(a, a.drec[q], nvf, pin));
         __syncthreads();
         const int kend = min(e, c1);
         for (; k + 4 <= kend; k += 4) {
@@ -582,6 +691,9 @@ static SolveArgs solve_args(const PbdBufs& b, const PbdParams& pp, const NarrowB
   a.lra_flag = b.lra_flag;
   a.vprefix = b.vprefix;
   a.vslot = b.vslot;
+  a.tstart = b.tstart;
+  a.eprefix = b.eprefix;
+  a.entries = b.entries;
   a.sid = b.sid;
   a.s_l0kp = b.s_l0kp;
   a.bid = b.bid;
@@ -599,11 +711,14 @@ static SolveArgs solve_args(const PbdBufs& b, const PbdParams& pp, const NarrowB
 
 // `iters` Jacobi iterations ping-ponging p -> p2 -> p ...
 static int launch_iterations(const PbdBufs& b, const SolveArgs& a, int iters, hipStream_t s) {
-  const int g = (b.n + kSolveV - 1) / kSolveV;
   float4* src = b.p;
   float4* dst = b.p2;
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_solve, dim3(g), dim3(kBlock), 0, s, a, src, dst);
+    if (b.ntiles > 0)
+      hipLaunchKernelGGL(k_solve<true>, dim3(b.ntiles), dim3(kBlock), 0, s, a, src, dst);
+    else
+      hipLaunchKernelGGL(k_solve<false>, dim3((b.n + kSolveV - 1) / kSolveV), dim3(kBlock), 0, s,
+                         a, src, dst);
     std::swap(src, dst);
   }
   CM_LAUNCH_CHECK();

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -1090,6 +1091,13 @@ struct Sim {
     }
   }
   // Physics/CudaPBD.cu:1024-1147 evaluated in arithmetic T on the state's values
+  // v / s per component: FP32 divides like the reference; the FP64 form (CM_F_BEND_FP64,
+  // this project's own precision choice) multiplies by one reciprocal per length.
+  template <class T>
+  static V3<T> bdiv(V3<T> v, T s) {
+    if constexpr (std::is_same<T, double>::value) return mul(v, T(1) / s);
+    else return divs(v, s);
+  }
   template <class T>
   void bend_impl(int bi, const std::vector<RV>& ps, V3<T> c[4]) const {
     typedef V3<T> TV;
@@ -1108,7 +1116,7 @@ struct Sim {
     T l23 = len(c23), l24 = len(c24);
     const T eps = T(1e-8f);
     if (l23 <= eps || l24 <= eps) return;
-    TV n1 = divs(c23, l23), n2 = divs(c24, l24);
+    TV n1 = bdiv(c23, l23), n2 = bdiv(c24, l24);
     T d = fclamp(dot(n1, n2), T(-1), T(1));
     T phi = std::acos(d);
     const T PIf = T(3.14159265359f);
@@ -1126,10 +1134,10 @@ struct Sim {
       phi = 0;
       if (phi == phi0) return;
     }
-    TV q3 = divs(cross(p2, n2) + mul(cross(n1, p2), d), l23);
-    TV q4 = divs(cross(p2, n1) + mul(cross(n2, p2), d), l24);
-    TV q2 = -divs(cross(p3, n2) + mul(cross(n1, p3), d), l23) -
-            divs(cross(p4, n1) + mul(cross(n2, p4), d), l24);
+    TV q3 = bdiv(cross(p2, n2) + mul(cross(n1, p2), d), l23);
+    TV q4 = bdiv(cross(p2, n1) + mul(cross(n2, p2), d), l24);
+    TV q2 = -bdiv(cross(p3, n2) + mul(cross(n1, p3), d), l23) -
+            bdiv(cross(p4, n1) + mul(cross(n2, p4), d), l24);
     TV q1 = -q2 - q3 - q4;
     T term = std::sqrt(T(1) - d * d) * (phi - phi0) * kp;
     T s1 = dot(q1, q1), s2 = dot(q2, q2), s3 = dot(q3, q3), s4 = dot(q4, q4);

@@ -161,3 +161,18 @@ def test_graph_replay_matches_eager_state(oracle_mod):
         b.fixed_update(1)
     assert np.array_equal(a.positions(), b.positions())
     assert np.array_equal(a.buffer("tri"), b.buffer("tri"))
+
+
+def test_high_valence_fan_slot_parallel_solve(oracle_mod):
+    """A fan whose centre holds more constraint slots (3,400) than a Jacobi tile
+    (3,072), so the solver runs its slot-parallel form instead of tiles."""
+    m = 1700
+    ang = np.linspace(0.0, 1.9 * np.pi, m + 1)
+    rim = np.stack([0.3 + 0.05 * np.cos(ang), np.full_like(ang, 0.4), 0.3 + 0.05 * np.sin(ang)], 1)
+    xyz = np.vstack([[0.3, 0.4, 0.3], rim]).astype(np.float32)
+    tri = np.array([[0, i, i + 1] for i in range(1, m + 1)], dtype=np.uint32)
+    params = dict(configs.config_scene(0, 8)["params"])
+    params["flags"] = configs.F_BEND_FP64
+    inv_mass = np.full(len(xyz), 1000.0, np.float32)
+    sc = configs._scene(params, xyz, tri, inv_mass, name="fan")
+    res, w = run_scene(oracle_mod, sc, 3)
