@@ -619,9 +619,9 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = dalloc(b, &b->lb.rmlr, t)) ||
       (rc = dalloc(b, (char**)&b->lb_temp, lbvh_build_temp_bytes(t))) ||
       (rc = dalloc(b, &b->chunk, pcap)) || (rc = dalloc(b, &b->pairs, pcap)) ||
-      (rc = dalloc(b, &b->leaf_cnt, t)) || (rc = dalloc(b, &b->leaf_off, t)) ||
-      (rc = dalloc(b, (char**)&b->scan_temp, scan_temp_bytes(0))))
+      (rc = dalloc(b, &b->leaf_cnt, t)) || (rc = dalloc(b, &b->leaf_off, t)))
     return fail(rc);
+  CM_HIP(hipMemset(b->sort_temp, 0, radix_sort_temp_bytes(t)));
   carve_build(t, b->lb_temp, &b->lb);
   CM_HIP(hipMemset(b->lb_temp, 0, lbvh_build_temp_bytes(t)));
   CM_HIP(hipMemset(b->lb.parents, 0, sizeof(int) * m));
@@ -652,6 +652,12 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   nb.d_nee = b->counters + 4;
   nb.d_nrec = b->counters + 5;
   nb.warn = b->warn;
+  {
+    // one scan temp for every scan of the substep, sized for the longest
+    const int64_t smax = std::max<int64_t>({(int64_t)t, (int64_t)n, pcap, nb.vf_cap, nb.ee_cap});
+    if ((rc = dalloc(b, (char**)&b->scan_temp, scan_temp_bytes((int)smax)))) return fail(rc);
+    CM_HIP(hipMemset(b->scan_temp, 0, scan_temp_bytes((int)smax)));
+  }
   nb.scan_temp = b->scan_temp;
   if ((rc = dalloc(b, &nb.mask, pcap)) || (rc = dalloc(b, &nb.cnt_vf, pcap)) ||
       (rc = dalloc(b, &nb.cnt_ee, pcap)) || (rc = dalloc(b, &nb.cand_vf, nb.vf_cap)) ||

@@ -34,6 +34,16 @@ void set_error(const std::string& msg);
   } while (0)
 
 constexpr int kBlock = 256;  // 4 wave64s
+constexpr int kXcds = 8;     // MI355X: workgroup i runs on XCD i mod 8
+
+// Logical block for hardware block b of n so that each XCD gets a contiguous range
+// of logical blocks (and so of data) instead of every eighth one. A bijection on
+// [0, n).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int q = n / kXcds, r = n % kXcds;  // XCD x holds q + (x < r) blocks
+  const int x = b % kXcds, k = b / kXcds;
+  return x * q + min(x, r) + k;
+}
 inline int nblocks(int64_t n, int block = kBlock) {
   int64_t b = (n + block - 1) / block;
   return (int)(b < 1 ? 1 : b);
@@ -42,12 +52,15 @@ inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
 // ---- primitives (k_prims.hip) --------------------------------------------
 size_t scan_temp_bytes(int n_max);
+// zero a scan temp before its first use (scans leave it zeroed for the next one)
+int scan_temp_clear(void* temp, int n_max, hipStream_t s);
 // exclusive prefix sum of in[0, n) into out[0, n), n = d_n ? min(*d_n, n_max) : n_max;
 // *d_total (if non-null) receives the sum. out may alias in.
 int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_total, void* temp,
                    hipStream_t s);
 size_t radix_sort_temp_bytes(int n);
 int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s);
+int radix_sort_clear(void* temp, int n, hipStream_t s);  // before the first sort on a temp
 int fill_i32(int* p, int v, int64_t n, hipStream_t s);
 int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 

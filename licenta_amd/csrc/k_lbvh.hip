@@ -437,7 +437,9 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __rest
                                                      int* __restrict__ counts,
                                                      uint32_t* __restrict__ warn) {
   __shared__ int sstk[kLdsStack][kBlock];
-  const int leaf = blockIdx.x * kBlock + threadIdx.x;
+  // Workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous eighth
+  // of the (Morton-ordered) leaves so its L2 holds the part of the tree it walks.
+  const int leaf = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (leaf >= t) return;
   const int nin = t - 1;
   const int me = nin + leaf;
@@ -720,6 +722,8 @@ extern "C" int cm_lbvh_sort(int32_t t, int32_t key_bits, uint64_t* d_keys, uint3
     set_error("cm_lbvh_sort: temp buffer too small");
     return CM_ERR_ARG;
   }
+  int rc = radix_sort_clear(d_temp, t, (hipStream_t)stream);
+  if (rc) return rc;
   return radix_sort(d_keys, d_vals, t, key_bits, d_temp, (hipStream_t)stream);
 }
 
@@ -802,6 +806,8 @@ extern "C" int cm_lbvh_pairs(int32_t t, const int32_t* d_lefts, const int32_t* d
   b.nodes = (float4*)tp;
   tp += align_up(4 * sizeof(float4) * (size_t)t);
   int rc = launch_pack_nodes(b, (hipStream_t)stream);
+  if (rc) return rc;
+  rc = scan_temp_clear(tp, t, (hipStream_t)stream);
   if (rc) return rc;
   return launch_pairs(b, cap, chunk, counts, offsets, (int2*)d_pairs, d_n_pairs, tp,
                       (hipStream_t)stream);

@@ -1,6 +1,6 @@
 // Device-wide primitives of libclothmi, written for CDNA4 wave64:
-//  * exclusive int32 scan over a device-side element count (reduce / scan
-//    partials / rescan, fixed grid so the sequence is graph-capturable);
+//  * single-pass exclusive int32 scan over a device-side element count
+//    (decoupled look-back, one launch, graph-capturable);
 //  * stable LSD radix sort of (uint64 key, uint32 value) with 8-bit digits.
 // They replace the CUB primitives the reference calls through
 // LicentaV2/LicentaV2/Core/CubWrappers.cuh:24-183 (DeviceScan::ExclusiveSum,
@@ -11,8 +11,6 @@
 namespace cm {
 
 namespace {
-
-constexpr int kScanGrid = 256;  // blocks of the reduce / rescan kernels
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -54,60 +52,117 @@ __device__ __forceinline__ int block_excl_scan(int v, int* lds, int& total) {
   return res;
 }
 
-__device__ __forceinline__ void scan_range(int n_max, const int* d_n, int& n, int& begin,
-                                           int& end) {
-  n = d_n ? min(*d_n, n_max) : n_max;
-  if (n < 0) n = 0;
-  int chunk = (n + kScanGrid - 1) / kScanGrid;
-  chunk = (chunk + kBlock - 1) / kBlock * kBlock;
-  begin = min((int)blockIdx.x * chunk, n);
-  end = min(begin + chunk, n);
+// ---- single-pass exclusive scan -------------------------------------------------
+// Tiles of kScanTile elements are taken in order from a ticket counter; each tile
+// publishes its sum, looks back over its predecessors (one wave reads 64 tile
+// states at a time) until it meets an inclusive prefix, then publishes its own
+// inclusive prefix and writes its outputs. A tile state is one 64-bit word (flag
+// in bits 62-63: 0 empty, 1 tile sum, 2 inclusive prefix; value in bits 0-31)
+// written and read with device-scope relaxed atomics, so no cache-wide fences are
+// needed. The last workgroup to finish clears the states and counters, leaving the
+// temp ready for the next scan on the stream.
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;  // 4096
+constexpr int kScanMaxGrid = 256;
+constexpr unsigned long long kFlagAgg = 1ull << 62, kFlagInc = 2ull << 62;
+
+__device__ __forceinline__ unsigned long long ld_state(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_state(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int* __restrict__ in, int n_max,
-                                                        const int* __restrict__ d_n,
-                                                        int* __restrict__ partials) {
-  __shared__ int lds[kBlock / 64];
-  int n, begin, end;
-  scan_range(n_max, d_n, n, begin, end);
-  int s = 0;
-  for (int i = begin + threadIdx.x; i < end; i += kBlock) s += in[i];
-  s = wave_reduce(s);
-  if (lane_id() == 0) lds[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
-    partials[blockIdx.x] = t;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_partials(int* __restrict__ partials,
-                                                          int* __restrict__ d_total) {
+__global__ void __launch_bounds__(kBlock) k_scan(const int* in, int* out, int n_max,
+                                                 const int* __restrict__ d_n,
+                                                 int* __restrict__ d_total, int* __restrict__ ctr,
+                                                 unsigned long long* __restrict__ state) {
   __shared__ int lds[kBlock / 64 + 1];
-  // kScanGrid == kBlock: one partial per thread
-  int v = partials[threadIdx.x];
-  int total;
-  int ex = block_excl_scan(v, lds, total);
-  partials[threadIdx.x] = ex;
-  if (threadIdx.x == 0 && d_total) *d_total = total;
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_apply(const int* __restrict__ in,
-                                                       int* __restrict__ out, int n_max,
-                                                       const int* __restrict__ d_n,
-                                                       const int* __restrict__ partials) {
-  __shared__ int lds[kBlock / 64 + 1];
-  int n, begin, end;
-  scan_range(n_max, d_n, n, begin, end);
-  int base = partials[blockIdx.x];
-  for (int tile = begin; tile < end; tile += kBlock) {
-    int i = tile + threadIdx.x;
-    int v = i < end ? in[i] : 0;
+  __shared__ int s_tile, s_prefix, s_last;
+  const int n = max(0, d_n ? min(*d_n, n_max) : n_max);
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  const int lane = lane_id();
+  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0 && d_total) *d_total = 0;
+  const bool vec = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[0], 1);
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile >= ntiles) break;
+    const int base = tile * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    if (vec && base + kScanItems <= n) {
+#pragma unroll
+      for (int q = 0; q < kScanItems / 4; ++q) {
+        const int4 t4 = reinterpret_cast<const int4*>(in + base)[q];
+        v[4 * q] = t4.x; v[4 * q + 1] = t4.y; v[4 * q + 2] = t4.z; v[4 * q + 3] = t4.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? in[base + k] : 0;
+    }
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += v[k];
     int total;
-    int ex = block_excl_scan(v, lds, total);
-    if (i < end) out[i] = base + ex;
-    base += total;
+    const int ex = block_excl_scan(sum, lds, total);
+    if (threadIdx.x < 64) {
+      int prefix = 0;
+      if (tile > 0) {
+        if (lane == 0) st_state(&state[tile], kFlagAgg | (unsigned)total);
+        int j = tile - 1;
+        for (;;) {
+          const int idx = j - lane;  // lane 0 is the nearest predecessor
+          const unsigned long long st = idx >= 0 ? ld_state(&state[idx]) : kFlagInc;
+          const unsigned flag = (unsigned)(st >> 62);
+          const unsigned long long pm = __ballot(flag == 2u), zm = __ballot(flag == 0u);
+          const int first = pm ? __ffsll((long long)pm) - 1 : 64;
+          const unsigned long long need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
+          if (zm & need) continue;  // a predecessor up to the first prefix is still empty
+          prefix += wave_reduce(lane <= first ? (int)(unsigned)st : 0);
+          if (first < 64) break;
+          j -= 64;
+        }
+      }
+      if (lane == 0) {
+        st_state(&state[tile], kFlagInc | (unsigned)(prefix + total));
+        s_prefix = prefix;
+      }
+    }
+    __syncthreads();
+    int run = s_prefix + ex;
+    if (vec && base + kScanItems <= n) {
+#pragma unroll
+      for (int q = 0; q < kScanItems / 4; ++q) {
+        int4 o;
+        o.x = run; run += v[4 * q];
+        o.y = run; run += v[4 * q + 1];
+        o.z = run; run += v[4 * q + 2];
+        o.w = run; run += v[4 * q + 3];
+        reinterpret_cast<int4*>(out + base)[q] = o;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) {
+          out[base + k] = run;
+          run += v[k];
+        }
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0 && d_total) *d_total = s_prefix + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(&ctr[1], 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    for (int i = threadIdx.x; i < ntiles; i += kBlock) st_state(&state[i], 0ull);
+    if (threadIdx.x == 0) {
+      ctr[0] = 0;
+      ctr[1] = 0;
+    }
   }
 }
 
@@ -202,15 +257,25 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(
 
 }  // namespace
 
-size_t scan_temp_bytes(int /*n_max*/) { return align_up(sizeof(int) * kScanGrid); }
+static int scan_tiles(int64_t n_max) {
+  return (int)std::max<int64_t>(1, (n_max + kScanTile - 1) / kScanTile);
+}
+
+size_t scan_temp_bytes(int n_max) {
+  return 256 + align_up(sizeof(unsigned long long) * (size_t)scan_tiles(n_max));
+}
+
+int scan_temp_clear(void* temp, int n_max, hipStream_t s) {
+  return fill_i32((int*)temp, 0, (int64_t)(scan_temp_bytes(n_max) / sizeof(int)), s);
+}
 
 int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_total, void* temp,
                    hipStream_t s) {
-  int* partials = (int*)temp;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(kScanGrid), dim3(kBlock), 0, s, in, n_max, d_n, partials);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, partials, d_total);
-  hipLaunchKernelGGL(k_scan_apply, dim3(kScanGrid), dim3(kBlock), 0, s, in, out, n_max, d_n,
-                     partials);
+  int* ctr = (int*)temp;
+  unsigned long long* state = (unsigned long long*)((char*)temp + 256);
+  const int grid = std::min(scan_tiles(n_max), kScanMaxGrid);
+  hipLaunchKernelGGL(k_scan, dim3(grid), dim3(kBlock), 0, s, in, out, n_max, d_n, d_total, ctr,
+                     state);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -232,6 +297,14 @@ size_t radix_sort_temp_bytes(int n) {
   int nb = sort_blocks(n);
   return align_up(sizeof(uint64_t) * (size_t)n) + align_up(sizeof(uint32_t) * (size_t)n) +
          align_up(sizeof(int) * 256 * (size_t)nb) + scan_temp_bytes(256 * nb);
+}
+
+// The scan part of a sort temp must start zeroed (each scan leaves it zeroed).
+int radix_sort_clear(void* temp, int n, hipStream_t s) {
+  const int nb = sort_blocks(n);
+  char* tp = (char*)temp + align_up(sizeof(uint64_t) * (size_t)n) +
+             align_up(sizeof(uint32_t) * (size_t)n) + align_up(sizeof(int) * 256 * (size_t)nb);
+  return scan_temp_clear(tp, 256 * nb, s);
 }
 
 int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s) {
