@@ -651,6 +651,7 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   nb.d_nvf = b->counters + 3;
   nb.d_nee = b->counters + 4;
   nb.d_nrec = b->counters + 5;
+  nb.d_ntotal = b->counters + 6;
   nb.warn = b->warn;
   {
     // one scan temp for every scan of the substep, sized for the longest
@@ -662,7 +663,11 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   if ((rc = dalloc(b, &nb.mask, pcap)) || (rc = dalloc(b, &nb.cnt_vf, pcap)) ||
       (rc = dalloc(b, &nb.cnt_ee, pcap)) || (rc = dalloc(b, &nb.cand_vf, nb.vf_cap)) ||
       (rc = dalloc(b, &nb.cand_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.flag_vf, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.vf_v, nb.vf_out_cap)) ||
+      (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.cplan, nb.vf_cap + nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.ccp, nb.vf_cap + nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.croot, 3 * (nb.vf_cap + nb.ee_cap))) ||
+      (rc = dalloc(b, &nb.vf_v, nb.vf_out_cap)) ||
       (rc = dalloc(b, &nb.vf_w, nb.vf_out_cap)) || (rc = dalloc(b, &nb.vf_tn, nb.vf_out_cap)) ||
       (rc = dalloc(b, &nb.ee_v, nb.ee_out_cap)) || (rc = dalloc(b, &nb.ee_w, nb.ee_out_cap)) ||
       (rc = dalloc(b, &nb.ee_tn, nb.ee_out_cap)) || (rc = dalloc(b, &nb.dcount, (size_t)n + 1)) ||

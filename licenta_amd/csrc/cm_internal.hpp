@@ -108,6 +108,12 @@ struct NarrowBufs {
   int4 *cand_vf = nullptr, *cand_ee = nullptr;
   int *d_ncand_vf = nullptr, *d_ncand_ee = nullptr;
   int *flag_vf = nullptr, *flag_ee = nullptr;  // per candidate (scanned in place)
+  // CCD passes, per candidate (VF then EE, vf_cap + ee_cap): plan word, critical
+  // points, and the secant root of each of its (up to three) intervals
+  int* cplan = nullptr;
+  float2* ccp = nullptr;
+  float* croot = nullptr;  // 3 * (vf_cap + ee_cap)
+  int* d_ntotal = nullptr;
   int4 *vf_v = nullptr, *ee_v = nullptr;      // compacted contacts
   float4 *vf_w = nullptr, *vf_tn = nullptr, *ee_w = nullptr, *ee_tn = nullptr;
   int *d_nvf = nullptr, *d_nee = nullptr;

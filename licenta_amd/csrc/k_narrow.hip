@@ -133,14 +133,17 @@ __device__ __forceinline__ float secant(f3 x21, f3 x31, f3 x41, f3 v21, f3 v31, 
   return -15.f;
 }
 
-// Coplanarity times in [0, 1] (Physics/CudaCubicSolver.cu:5-153), handed to
-// on_root in the reference's order; stops at the first root on_root accepts.
-// The reference collects every root first and then tests them in order until one
-// hits, so stopping early gives the same contact. f at an interval end is shared
-// with the next interval's start.
-template <class OnRoot>
-__device__ __forceinline__ bool coplanarity_times(f3 x1, f3 x2, f3 x3, f3 x4, f3 v1, f3 v2,
-                                                  f3 v3, f3 v4, OnRoot&& on_root) {
+// The coplanarity cubic of Physics/CudaCubicSolver.cu:5-153: relative positions and
+// velocities of vertices 2-4 against vertex 1, and the critical points in [0, 1]
+// that split [0, 1] into cc + 1 intervals (ends 0, [cp1], [cp2], 1).
+struct Cubic {
+  f3 x21, x31, x41, v21, v31, v41;
+  float cp1, cp2;
+  int cc;
+};
+
+__device__ __forceinline__ void cubic_setup(f3 x1, f3 x2, f3 x3, f3 x4, f3 v1, f3 v2, f3 v3,
+                                            f3 v4, Cubic& q) {
   const f3 x21 = sub(x2, x1), x31 = sub(x3, x1), x41 = sub(x4, x1);
   const f3 v21 = sub(v2, v1), v31 = sub(v3, v1), v41 = sub(v4, v1);
   const f3 v21v31 = cross(v21, v31), x21v31 = cross(x21, v31), v21x31 = cross(v21, x31),
@@ -167,21 +170,18 @@ __device__ __forceinline__ bool coplanarity_times(f3 x1, f3 x2, f3 x3, f3 x4, f3
     if (between(t1, 0.f, 1.f)) { cp1 = t1; have1 = true; ++cc; }
   }
   if (cc == 1 && !have1) cp1 = cp2;
-  // interval ends: 0, [cp1], [cp2], 1
-  float ea = 0.f;
-  float fa = triple_scalar(x21, x31, x41, v21, v31, v41, ea);
-  for (int i = 0; i <= cc; ++i) {
-    const float eb = i == cc ? 1.f : (i == 0 ? cp1 : cp2);
-    const float fb = triple_scalar(x21, x31, x41, v21, v31, v41, eb);
-    if (fabsf(fa) < kCubicEps && on_root(ea)) return true;
-    if (fa * fb < 0.f) {
-      const float tt = secant(x21, x31, x41, v21, v31, v41, ea, eb);
-      if (tt != -15 && on_root(tt)) return true;
-    }
-    ea = eb;
-    fa = fb;
-  }
-  return false;
+  q.x21 = x21; q.x31 = x31; q.x41 = x41;
+  q.v21 = v21; q.v31 = v31; q.v41 = v41;
+  q.cp1 = cp1; q.cp2 = cp2; q.cc = cc;
+}
+
+// end of interval i (its start is the end of interval i - 1, or 0)
+__device__ __forceinline__ float cubic_end(float cp1, float cp2, int cc, int i) {
+  return i == cc ? 1.f : (i == 0 ? cp1 : cp2);
+}
+
+__device__ __forceinline__ float cubic_f(const Cubic& q, float t) {
+  return triple_scalar(q.x21, q.x31, q.x41, q.v21, q.v31, q.v41, t);
 }
 
 struct Hit {
@@ -283,85 +283,277 @@ __device__ __forceinline__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, floa
   return false;
 }
 
-// CCD of one candidate (Collision/CudaPrimitiveTests.cu:128-238) on prev positions x
-// and velocities v; h receives the contact data when it hits.
-__device__ __forceinline__ bool ccd_candidate(bool isvf, int4 c, const float4* __restrict__ x,
-                              const float4* __restrict__ v, float th, float dt, Hit& h) {
+// ---- CCD in passes ------------------------------------------------------------
+// The reference tests each candidate (Collision/CudaPrimitiveTests.cu:128-238): it
+// finds the coplanarity roots in [0, 1] in interval order (a root at an interval
+// start when |f| < EPS there, then a secant root when f changes sign), advances all
+// four positions cumulatively by root * dt, and runs the VF / EE proximity test
+// after each root until one hits. The cubic depends on x and v only, so the secant
+// roots are independent of the tests and are found here in a pass of their own:
+//   k_ccd_plan    per candidate: cubic, f at the interval ends, plan word
+//   k_ccd_secant  every wave runs the secant searches of its range of candidates,
+//                 each lane taking the next candidate when its searches end, so a
+//                 long search does not idle the others; root of interval i of
+//                 candidate id in roots[3 * id + i]
+//   k_ccd_test    per candidate: the roots in order, cumulative advance, tests
+// Plan word: bits 0-1 cc, bits 2-4 root at the start of interval i, bits 5-7 sign
+// change on interval i, bit 8 nothing to test (degenerate candidate).
+constexpr int kPlanDead = 1 << 8;
+constexpr int kPlanSecants = 7 << 5;
+constexpr float kNoRoot = -15.f;  // Physics/CudaCubicSolver.cu:152
+
+// Vertex ids of candidate c in the cubic's order (x1, x2, x3, x4): VF candidates are
+// (point, triangle) and the cubic takes the triangle first; EE as stored.
+__device__ __forceinline__ int4 cubic_ids(bool isvf, int4 c) {
+  return isvf ? make_int4(c.y, c.z, c.w, c.x) : c;
+}
+__device__ __forceinline__ bool sane(bool isvf, int4 c) {
+  return isvf ? !(c.x == c.y || c.x == c.z || c.x == c.w)                    // VFSanity
+              : !(c.x == c.z || c.x == c.w || c.y == c.z || c.y == c.w);   // EESanity
+}
+
+__device__ __forceinline__ int4 cand_of(int id, int nvf, const int4* __restrict__ cand_vf,
+                                        const int4* __restrict__ cand_ee, bool& isvf) {
+  isvf = id < nvf;
+  return isvf ? cand_vf[id] : cand_ee[id - nvf];
+}
+
+__global__ void __launch_bounds__(kBlock) k_ccd_plan(
+    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
+    const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
+    int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v,
+    int* __restrict__ plan, float2* __restrict__ cps, int* __restrict__ d_ntotal) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
+  const int total = nvf + nee;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d_ntotal = total;
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total; id += gridDim.x * kBlock) {
+    bool isvf;
+    const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
+    int pl = kPlanDead;
+    float2 cp = make_float2(0.f, 0.f);
+    if (sane(isvf, c)) {
+      const int4 q4 = cubic_ids(isvf, c);
+      Cubic q;
+      cubic_setup(xyz(x[q4.x]), xyz(x[q4.y]), xyz(x[q4.z]), xyz(x[q4.w]), xyz(v[q4.x]),
+                  xyz(v[q4.y]), xyz(v[q4.z]), xyz(v[q4.w]), q);
+      pl = q.cc;
+      float fa = cubic_f(q, 0.f);
+      for (int i = 0; i <= q.cc; ++i) {
+        const float fb = cubic_f(q, cubic_end(q.cp1, q.cp2, q.cc, i));
+        if (fabsf(fa) < kCubicEps) pl |= 1 << (2 + i);
+        if (fa * fb < 0.f) pl |= 1 << (5 + i);
+        fa = fb;
+      }
+      cp = make_float2(q.cp1, q.cp2);
+    }
+    plan[id] = pl;
+    cps[id] = cp;
+  }
+}
+
+// Secant searches (Physics/CudaCubicSolver.cu:131-153; f(x0) carried over from the
+// previous step, see secant()). Wave w owns candidates [w*per, (w+1)*per); whenever
+// at least kRefill lanes are idle they take the next candidates of the range
+// together, so most steps of the loop are one secant iteration on most lanes.
+constexpr int kRefill = 24;
+
+__global__ void __launch_bounds__(kBlock) k_ccd_secant(
+    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
+    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
+    const int* __restrict__ plan, const float2* __restrict__ cps, const float4* __restrict__ x,
+    const float4* __restrict__ v, float* __restrict__ roots) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
+  const int lane = threadIdx.x & 63;
+  const int nwaves = gridDim.x * (kBlock / 64);
+  const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int per = (total + nwaves - 1) / nwaves;
+  int next = min(w * per, total);
+  const int end = min(next + per, total);
+  const uint64_t below = (1ull << lane) - 1ull;
+  bool active = false;
+  int id = 0, pl = 0, ivl = 0, it = 0;
+  float2 cp = make_float2(0.f, 0.f);
+  float x0 = 0.f, x1 = 0.f, f0 = 0.f;
+  Cubic q;
+  // next interval of the current candidate with a sign change, from ivl on
+  auto start_interval = [&]() {
+    while (ivl <= (pl & 3) && !(pl & (1 << (5 + ivl)))) ++ivl;
+    if (ivl > (pl & 3)) {
+      active = false;
+      return;
+    }
+    x0 = ivl == 0 ? 0.f : cubic_end(cp.x, cp.y, pl & 3, ivl - 1);
+    x1 = cubic_end(cp.x, cp.y, pl & 3, ivl);
+    f0 = cubic_f(q, x0);
+    it = 0;
+  };
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const int nidle = __popcll(idle);
+    if (next < end && (nidle >= kRefill || nidle == 64)) {
+      const int r = __popcll(idle & below);
+      if (!active && next + r < end) {
+        id = next + r;
+        pl = plan[id];
+        if (!(pl & kPlanDead) && (pl & kPlanSecants)) {
+          bool isvf;
+          const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
+          const int4 q4 = cubic_ids(isvf, c);
+          cp = cps[id];
+          const f3 a1 = xyz(x[q4.x]), b1 = xyz(v[q4.x]);
+          q.x21 = sub(xyz(x[q4.y]), a1);
+          q.x31 = sub(xyz(x[q4.z]), a1);
+          q.x41 = sub(xyz(x[q4.w]), a1);
+          q.v21 = sub(xyz(v[q4.y]), b1);
+          q.v31 = sub(xyz(v[q4.z]), b1);
+          q.v41 = sub(xyz(v[q4.w]), b1);
+          ivl = 0;
+          active = true;
+          start_interval();
+        }
+      }
+      next = min(next + nidle, end);
+    }
+    if (!__any(active)) {
+      if (next >= end) break;
+      continue;
+    }
+    if (active) {
+      const float f1 = cubic_f(q, x1);
+      const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
+      ++it;
+      const bool conv = fabsf(xc - x1) < kCubicEps;
+      if (conv || it == 20) {
+        roots[3 * (int64_t)id + ivl] = conv ? xc : kNoRoot;
+        ++ivl;
+        start_interval();
+      } else {
+        x0 = x1;
+        f0 = f1;
+        x1 = xc;
+      }
+    }
+  }
+}
+
+// Conservative rejections before the proximity tests. A VF hit puts the point within
+// th of the triangle (plane distance <= th, projection inside); an EE hit puts two
+// points of the segments (or their midpoints) within th. So a box gap larger than
+// th plus a rounding margin means the full test returns false.
+__device__ __forceinline__ float box_gap2(f3 amn, f3 amx, f3 bmn, f3 bmx) {
+  const float gx = fmaxf(fmaxf(bmn.x - amx.x, amn.x - bmx.x), 0.f);
+  const float gy = fmaxf(fmaxf(bmn.y - amx.y, amn.y - bmx.y), 0.f);
+  const float gz = fmaxf(fmaxf(bmn.z - amx.z, amn.z - bmx.z), 0.f);
+  return gx * gx + gy * gy + gz * gz;
+}
+__device__ __forceinline__ float reach2(float th) {
+  const float r = th * 1.0001f + 2e-5f;
+  return r * r;
+}
+__device__ __forceinline__ bool vf_far(f3 p, f3 a, f3 b, f3 c, float th) {
+  return box_gap2(p, p, min3(a, b, c), max3(a, b, c)) > reach2(th);
+}
+__device__ __forceinline__ bool ee_far(f3 a, f3 b, f3 c, f3 d, float th) {
+  return box_gap2(vmin(a, b), vmax(a, b), vmin(c, d), vmax(c, d)) > reach2(th);
+}
+
+// The reference's per-candidate test loop over the planned roots; true on a hit
+// (contact data in h).
+__device__ __forceinline__ bool ccd_tests(bool isvf, int4 c, int pl, float2 cp, int id,
+                                          const float* __restrict__ roots,
+                                          const float4* __restrict__ x,
+                                          const float4* __restrict__ v, float th, float dt,
+                                          Hit& h) {
   h.t = -1.f;
   h.w[0] = h.w[1] = h.w[2] = h.w[3] = -1.f;
   h.n = mk3(-1.f, -1.f, -1.f);
-  if (isvf) {
-    if (c.x == c.y || c.x == c.z || c.x == c.w) return false;  // VFSanity
-    f3 x4 = xyz(x[c.x]), x1 = xyz(x[c.y]), x2 = xyz(x[c.z]), x3 = xyz(x[c.w]);
-    const f3 v4 = xyz(v[c.x]), v1 = xyz(v[c.y]), v2 = xyz(v[c.z]), v3 = xyz(v[c.w]);
-    // positions advance cumulatively by each root (Collision/CudaPrimitiveTests.cu:164-171)
-    return coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, [&](float root) {
-      const float t = root * dt;
-      x4 = add(x4, mul(v4, t));
-      x1 = add(x1, mul(v1, t));
-      x2 = add(x2, mul(v2, t));
-      x3 = add(x3, mul(v3, t));
-      if (!test_vf(x4, x1, x2, x3, h, th)) return false;
-      h.t = t;
-      return true;
-    });
-  }
-  if (c.x == c.z || c.x == c.w || c.y == c.z || c.y == c.w) return false;  // EESanity
-  f3 x1 = xyz(x[c.x]), x2 = xyz(x[c.y]), x3 = xyz(x[c.z]), x4 = xyz(x[c.w]);
-  const f3 v1 = xyz(v[c.x]), v2 = xyz(v[c.y]), v3 = xyz(v[c.z]), v4 = xyz(v[c.w]);
-  return coplanarity_times(x1, x2, x3, x4, v1, v2, v3, v4, [&](float root) {
+  if (pl & kPlanDead) return false;
+  const int4 q4 = cubic_ids(isvf, c);
+  const f3 o1 = xyz(x[q4.x]), o2 = xyz(x[q4.y]), o3 = xyz(x[q4.z]), o4 = xyz(x[q4.w]);
+  const f3 u1 = xyz(v[q4.x]), u2 = xyz(v[q4.y]), u3 = xyz(v[q4.z]), u4 = xyz(v[q4.w]);
+  f3 p1 = o1, p2 = o2, p3 = o3, p4 = o4;  // advanced cumulatively by each root
+  // one root: advance (Collision/CudaPrimitiveTests.cu:164-171), then test
+  auto at_root = [&](float root) {
     const float t = root * dt;
-    x1 = add(x1, mul(v1, t));
-    x2 = add(x2, mul(v2, t));
-    x3 = add(x3, mul(v3, t));
-    x4 = add(x4, mul(v4, t));
-    if (!test_ee(x1, x2, x3, x4, h, th)) return false;
+    if (isvf) {  // the reference advances x4 (the point) first; the order is immaterial
+      p4 = add(p4, mul(u4, t));
+      p1 = add(p1, mul(u1, t));
+      p2 = add(p2, mul(u2, t));
+      p3 = add(p3, mul(u3, t));
+      if (vf_far(p4, p1, p2, p3, th) || !test_vf(p4, p1, p2, p3, h, th)) return false;
+    } else {
+      p1 = add(p1, mul(u1, t));
+      p2 = add(p2, mul(u2, t));
+      p3 = add(p3, mul(u3, t));
+      p4 = add(p4, mul(u4, t));
+      if (ee_far(p1, p2, p3, p4, th) || !test_ee(p1, p2, p3, p4, h, th)) return false;
+    }
     h.t = t;
     return true;
-  });
+  };
+  const int cc = pl & 3;
+  float ea = 0.f;
+  for (int i = 0; i <= cc; ++i) {
+    if ((pl & (1 << (2 + i))) && at_root(ea)) return true;
+    if (pl & (1 << (5 + i))) {
+      const float r = roots[3 * (int64_t)id + i];
+      if (r != kNoRoot && at_root(r)) return true;
+    }
+    ea = cubic_end(cp.x, cp.y, cc, i);
+  }
+  return false;
 }
 
-// pass 1: hit flags for every candidate (VF then EE)
-__global__ void __launch_bounds__(kBlock) k_ccd(
+// pass 4: hit flags (VF then EE candidates)
+__global__ void __launch_bounds__(kBlock) k_ccd_test(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
-    const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
-    int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
-    float dt, int* __restrict__ flag_vf, int* __restrict__ flag_ee) {
-  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < nvf + nee; id += gridDim.x * kBlock) {
-    const bool isvf = id < nvf;
-    const int k = isvf ? id : id - nvf;
+    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
+    const int* __restrict__ plan, const float2* __restrict__ cps,
+    const float* __restrict__ roots, const float4* __restrict__ x, const float4* __restrict__ v,
+    float th, float dt, int* __restrict__ flag_vf, int* __restrict__ flag_ee) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total; id += gridDim.x * kBlock) {
+    bool isvf;
+    const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
     Hit h;
-    const bool hit = ccd_candidate(isvf, isvf ? cand_vf[k] : cand_ee[k], x, v, th, dt, h);
-    if (isvf) flag_vf[k] = hit; else flag_ee[k] = hit;
+    const bool hit = ccd_tests(isvf, c, plan[id], cps[id], id, roots, x, v, th, dt, h);
+    if (isvf) flag_vf[id] = hit; else flag_ee[id - nvf] = hit;
   }
 }
 
-// pass 2: flags were scanned in place into offsets (total in *d_n); each hit is
+// pass 5: flags were scanned in place into offsets (total in *d_n); each hit is
 // recomputed and written at its compacted position, keeping candidate order.
 __global__ void __launch_bounds__(kBlock) k_ccd_write(
-    bool isvf, const int4* __restrict__ cand, const int* __restrict__ d_ncand, int64_t cap,
-    const int* __restrict__ off, const int* __restrict__ d_n, int64_t out_cap,
-    const float4* __restrict__ x, const float4* __restrict__ v, float th, float dt,
-    int4* __restrict__ ov, float4* __restrict__ ow, float4* __restrict__ otn,
-    uint32_t* __restrict__ warn) {
-  const int nc = (int)min((int64_t)*d_ncand, cap);
-  const int total = *d_n;
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nc; k += gridDim.x * kBlock) {
+    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
+    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
+    const int* __restrict__ plan, const float2* __restrict__ cps,
+    const float* __restrict__ roots, const int* __restrict__ off_vf,
+    const int* __restrict__ d_hvf, const int* __restrict__ off_ee, const int* __restrict__ d_hee,
+    int64_t vf_out_cap, int64_t ee_out_cap, const float4* __restrict__ x,
+    const float4* __restrict__ v, float th, float dt, int4* __restrict__ vf_v,
+    float4* __restrict__ vf_w, float4* __restrict__ vf_tn, int4* __restrict__ ee_v,
+    float4* __restrict__ ee_w, float4* __restrict__ ee_tn, uint32_t* __restrict__ warn) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
+  const int hvf = *d_hvf, hee = *d_hee;
+  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total; id += gridDim.x * kBlock) {
+    const bool isvf = id < nvf;
+    const int k = isvf ? id : id - nvf;
+    const int* off = isvf ? off_vf : off_ee;
+    const int n = isvf ? nvf : total - nvf;
     const int o = off[k];
-    const int next = k + 1 < nc ? off[k + 1] : total;
-    if (next > o) {
-      if (o >= out_cap) {
-        atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
-        continue;
-      }
-      const int4 c = cand[k];
-      Hit h;
-      ccd_candidate(isvf, c, x, v, th, dt, h);
-      ov[o] = c;
-      ow[o] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
-      otn[o] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+    const int next = k + 1 < n ? off[k + 1] : (isvf ? hvf : hee);
+    if (next == o) continue;
+    if (o >= (isvf ? vf_out_cap : ee_out_cap)) {
+      atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
+      continue;
     }
+    const int4 c = isvf ? cand_vf[k] : cand_ee[k];
+    Hit h;
+    ccd_tests(isvf, c, plan[id], cps[id], id, roots, x, v, th, dt, h);
+    const float4 w4 = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+    const float4 tn = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+    if (isvf) { vf_v[o] = c; vf_w[o] = w4; vf_tn[o] = tn; }
+    else { ee_v[o] = c; ee_w[o] = w4; ee_tn[o] = tn; }
   }
 }
 
@@ -467,6 +659,8 @@ __global__ void k_min_cap(int* __restrict__ d_n, int64_t cap) {
 
 }  // namespace
 
+constexpr int kSecantGrid = 2048;  // persistent workgroups of the secant pass
+
 static int grid_for(int64_t cap) {
   int64_t g = (cap + kBlock - 1) / kBlock;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
@@ -489,9 +683,16 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
                      nb.vf_cap, nb.ee_cap, nb.warn);
   hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_vf, nb.vf_cap);
   hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_ee, nb.ee_cap);
-  hipLaunchKernelGGL(k_ccd, dim3(grid_for(nb.vf_cap + nb.ee_cap)), dim3(kBlock), 0, s, nb.cand_vf,
-                     nb.cand_ee, nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, th, dt,
-                     nb.flag_vf, nb.flag_ee);
+  const int64_t ccap = nb.vf_cap + nb.ee_cap;
+  const int gc = grid_for(ccap);
+  hipLaunchKernelGGL(k_ccd_plan, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
+                     nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, nb.cplan, nb.ccp,
+                     nb.d_ntotal);
+  hipLaunchKernelGGL(k_ccd_secant, dim3(kSecantGrid), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
+                     nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, x, v, nb.croot);
+  hipLaunchKernelGGL(k_ccd_test, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
+                     nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, nb.croot, x, v, th,
+                     dt, nb.flag_vf, nb.flag_ee);
   CM_LAUNCH_CHECK();
   rc = scan_exclusive(nb.flag_vf, nb.flag_vf, (int)nb.vf_cap, nb.d_ncand_vf, nb.d_nvf,
                       nb.scan_temp, s);
@@ -499,12 +700,10 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   rc = scan_exclusive(nb.flag_ee, nb.flag_ee, (int)nb.ee_cap, nb.d_ncand_ee, nb.d_nee,
                       nb.scan_temp, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_ccd_write, dim3(grid_for(nb.vf_cap)), dim3(kBlock), 0, s, true, nb.cand_vf,
-                     nb.d_ncand_vf, nb.vf_cap, nb.flag_vf, nb.d_nvf, nb.vf_out_cap, x, v, th, dt,
-                     nb.vf_v, nb.vf_w, nb.vf_tn, nb.warn);
-  hipLaunchKernelGGL(k_ccd_write, dim3(grid_for(nb.ee_cap)), dim3(kBlock), 0, s, false, nb.cand_ee,
-                     nb.d_ncand_ee, nb.ee_cap, nb.flag_ee, nb.d_nee, nb.ee_out_cap, x, v, th, dt,
-                     nb.ee_v, nb.ee_w, nb.ee_tn, nb.warn);
+  hipLaunchKernelGGL(k_ccd_write, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
+                     nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, nb.croot,
+                     nb.flag_vf, nb.d_nvf, nb.flag_ee, nb.d_nee, nb.vf_out_cap, nb.ee_out_cap, x,
+                     v, th, dt, nb.vf_v, nb.vf_w, nb.vf_tn, nb.ee_v, nb.ee_w, nb.ee_tn, nb.warn);
   hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_nvf, nb.vf_out_cap);
   hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_nee, nb.ee_out_cap);
   CM_LAUNCH_CHECK();
