@@ -522,7 +522,8 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
               int32_t t, const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
               int32_t n_dist, const float* k_dist, const float* k_bend, cm_body** out) {
   if (out) *out = nullptr;
-  if (!params || !xyz || n < 1 || !tri || t < 1 || !inv_mass || !out ||
+  // vertex ids share a word with 3 flag bits in the contact records
+  if (!params || !xyz || n < 1 || n >= (1 << 29) || !tri || t < 1 || !inv_mass || !out ||
       params->solver_iterations < 1 || (params->morton_bits != 30 && params->morton_bits != 63) ||
       params->pair_cap < 1 || (dist_pairs && n_dist < 0) || params->timestep <= 0.f) {
     set_error("cm_create: invalid argument");
@@ -674,6 +675,7 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
       (rc = dalloc(b, &nb.dprefix, (size_t)n + 1)) || (rc = dalloc(b, &nb.dcursor, n)) ||
       (rc = dalloc(b, &nb.drec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
       (rc = dalloc(b, &nb.dtmp, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
+      (rc = dalloc(b, &nb.rec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
       (rc = dalloc(b, &nb.dlong, (size_t)n)))
     return fail(rc);
   CM_HIP(hipMemset(nb.dprefix, 0, sizeof(int) * ((size_t)n + 1)));

@@ -96,6 +96,14 @@ int launch_gather_tris(int t, const uint32_t* perm, const int4* tri_in, int4* tr
 int launch_copy_i4(int64_t n, const int4* src, int4* dst, hipStream_t s);
 
 // ---- narrow phase (k_narrow.hip) ------------------------------------------
+// A contact record in vertex order, self-contained for the Jacobi kernel: the
+// contact's four vertex ids (bits 29-30 of ids.x hold this record's role, bit 31 is
+// set for a VF contact) and the contact weights.
+struct alignas(16) RecPayload {
+  int4 ids;
+  float4 w;
+};
+
 struct NarrowBufs {
   int n = 0, t = 0;
   int64_t pair_cap = 0;       // capacity of pairs (T*cap)
@@ -121,6 +129,7 @@ struct NarrowBufs {
   int *dcount = nullptr, *dprefix = nullptr, *dcursor = nullptr;  // n + 1, n + 1, n
   uint32_t* drec = nullptr;  // 4 * (vf_out_cap + ee_out_cap)
   uint32_t* dtmp = nullptr;  // same size: staging for very long record lists
+  RecPayload* rec = nullptr;  // same size: record payloads in drec order
   int* dlong = nullptr;      // n: vertices whose record lists are ordered by a workgroup
   int* d_nrec = nullptr;
   uint32_t* warn = nullptr;

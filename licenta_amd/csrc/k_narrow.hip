@@ -586,6 +586,23 @@ __global__ void __launch_bounds__(kBlock) k_dyn_fill(const int4* __restrict__ vf
   }
 }
 
+__device__ __forceinline__ void put_payload(uint32_t r, int k, int nvf,
+                                            const int4* __restrict__ vf_v,
+                                            const float4* __restrict__ vf_w,
+                                            const int4* __restrict__ ee_v,
+                                            const float4* __restrict__ ee_w,
+                                            RecPayload* __restrict__ out) {
+  const int ci = (int)(r >> 2), role = (int)(r & 3u);
+  const bool isvf = ci < nvf;
+  int4 ids = isvf ? vf_v[ci] : ee_v[ci - nvf];
+  const float4 w = isvf ? vf_w[ci] : ee_w[ci - nvf];
+  ids.x |= (role << 29) | (isvf ? (int)0x80000000u : 0);
+  RecPayload rp;
+  rp.ids = ids;
+  rp.w = w;
+  out[k] = rp;
+}
+
 constexpr int kShortList = 32;   // longer record lists are ordered by a workgroup
 constexpr int kLongLds = 4096;   // records a workgroup ranks in LDS
 
@@ -594,7 +611,13 @@ constexpr int kLongLds = 4096;   // records a workgroup ranks in LDS
 __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restrict__ prefix,
                                                      uint32_t* __restrict__ drec,
                                                      int* __restrict__ nlong,
-                                                     int* __restrict__ long_list) {
+                                                     int* __restrict__ long_list,
+                                                     const int* __restrict__ d_nvf,
+                                                     const int4* __restrict__ vf_v,
+                                                     const float4* __restrict__ vf_w,
+                                                     const int4* __restrict__ ee_v,
+                                                     const float4* __restrict__ ee_w,
+                                                     RecPayload* __restrict__ rec) {
   const int v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= n) return;
   const int b = prefix[v], e = prefix[v + 1];
@@ -611,6 +634,8 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restric
     }
     drec[j + 1] = key;
   }
+  const int nvf = *d_nvf;
+  for (int i = b; i < e; ++i) put_payload(drec[i], i, nvf, vf_v, vf_w, ee_v, ee_w, rec);
 }
 
 // one workgroup per long list: record indices are unique, so the rank of a record
@@ -619,9 +644,16 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort_long(const int* __restrict_
                                                           const int* __restrict__ nlong,
                                                           const int* __restrict__ long_list,
                                                           uint32_t* __restrict__ drec,
-                                                          uint32_t* __restrict__ tmp) {
+                                                          uint32_t* __restrict__ tmp,
+                                                          const int* __restrict__ d_nvf,
+                                                          const int4* __restrict__ vf_v,
+                                                          const float4* __restrict__ vf_w,
+                                                          const int4* __restrict__ ee_v,
+                                                          const float4* __restrict__ ee_w,
+                                                          RecPayload* __restrict__ rec) {
   __shared__ uint32_t sl[kLongLds];
   const int nl = *nlong;
+  const int nvf = *d_nvf;
   for (int w = blockIdx.x; w < nl; w += gridDim.x) {
     const int v = long_list[w];
     const int b = prefix[v], k = prefix[v + 1] - b;
@@ -634,6 +666,7 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort_long(const int* __restrict_
         int r = 0;
         for (int m = 0; m < k; ++m) r += sl[m] < key;
         drec[b + r] = key;
+        put_payload(key, b + r, nvf, vf_v, vf_w, ee_v, ee_w, rec);
       }
     } else {
       for (int j = threadIdx.x; j < k; j += kBlock) {
@@ -643,7 +676,10 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort_long(const int* __restrict_
         tmp[b + r] = key;
       }
       __syncthreads();
-      for (int j = threadIdx.x; j < k; j += kBlock) drec[b + j] = tmp[b + j];
+      for (int j = threadIdx.x; j < k; j += kBlock) {
+        drec[b + j] = tmp[b + j];
+        put_payload(tmp[b + j], b + j, nvf, vf_v, vf_w, ee_v, ee_w, rec);
+      }
     }
     __syncthreads();
   }
@@ -724,9 +760,11 @@ int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
   hipLaunchKernelGGL(k_dyn_fill, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
                      nb.d_nee, nb.dcursor, nb.drec);
   hipLaunchKernelGGL(k_dyn_sort, dim3(nblocks(nb.n)), dim3(kBlock), 0, s, nb.n, nb.dprefix,
-                     nb.drec, nb.dcount + nb.n, nb.dlong);
+                     nb.drec, nb.dcount + nb.n, nb.dlong, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v,
+                     nb.ee_w, nb.rec);
   hipLaunchKernelGGL(k_dyn_sort_long, dim3(512), dim3(kBlock), 0, s, nb.dprefix, nb.dcount + nb.n,
-                     nb.dlong, nb.drec, nb.dtmp);
+                     nb.dlong, nb.drec, nb.dtmp, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v, nb.ee_w,
+                     nb.rec);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

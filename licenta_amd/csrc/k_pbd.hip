@@ -442,10 +442,7 @@ struct SolveArgs {
   const int4* bid;
   const float2* b_kpphi0;
   const int* dprefix;
-  const uint32_t* drec;
-  const int* d_nvf;
-  const int4 *vf_v, *ee_v;
-  const float4 *vf_w, *ee_w;
+  const RecPayload* rec;  // dynamic records in per-vertex order
   // vertex tiles: tstart[ntiles + 1] vertex ranges, eprefix[ntiles + 1] entry ranges
   const int* tstart;
   const int* eprefix;
@@ -464,12 +461,12 @@ __device__ __forceinline__ f3 slot_corr(const SolveArgs& a, uint32_t s,
   return stretch_corr(a.sid[idx], a.s_l0kp[idx], role, pin);
 }
 
-// one dynamic record's correction: record word = 4 * contact + role, VF contacts first
-__device__ __forceinline__ f3 rec_corr(const SolveArgs& a, uint32_t r, int nvf,
+// one dynamic record's correction (payload: contact ids with role and VF bit, weights)
+__device__ __forceinline__ f3 rec_corr(const SolveArgs& a, const RecPayload& rp,
                                        const float4* __restrict__ pin) {
-  const int ci = (int)(r >> 2), role = (int)(r & 3u);
-  return ci < nvf ? vf_corr(a.vf_v[ci], a.vf_w[ci], role, a.th, pin)
-                  : ee_corr(a.ee_v[ci - nvf], a.ee_w[ci - nvf], role, a.th, pin);
+  const int role = (rp.ids.x >> 29) & 3;
+  const int4 ids = make_int4(rp.ids.x & 0x1fffffff, rp.ids.y, rp.ids.z, rp.ids.w);
+  return rp.ids.x < 0 ? vf_corr(ids, rp.w, role, a.th, pin) : ee_corr(ids, rp.w, role, a.th, pin);
 }
 
 constexpr int kSolveV = 64;  // vertices per workgroup (slot-parallel form)
@@ -589,7 +586,6 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
   if (a.dprefix) {
     const int r0 = a.dprefix[v0], r1 = a.dprefix[v1];
     if (r1 > r0) {
-      const int nvf = *a.d_nvf;
       int k = mine ? a.dprefix[i] : 0;
       const int e = mine ? a.dprefix[i + 1] : 0;
       const bool any = mine && e > k;
@@ -598,7 +594,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
       for (int c0 = r0; c0 < r1; c0 += kLds) {
         const int c1 = min(c0 + kLds, r1);
         for (int q = c0 + tid; q < c1; q += kBlock)
-          lds_put(cx, cy, cz, q - c0, rec_corr(a, a.drec[q], nvf, pin));
+          lds_put(cx, cy, cz, q - c0, rec_corr(a, a.rec[q], pin));
         __syncthreads();
         const int kend = min(e, c1);
         for (; k + 4 <= kend; k += 4) {
@@ -700,12 +696,7 @@ static SolveArgs solve_args(const PbdBufs& b, const PbdParams& pp, const NarrowB
   a.b_kpphi0 = b.b_kpphi0;
   const bool coll = (pp.flags & CM_F_SELF_COLLISION) && nb.dprefix;
   a.dprefix = coll ? nb.dprefix : nullptr;
-  a.drec = nb.drec;
-  a.d_nvf = nb.d_nvf;
-  a.vf_v = nb.vf_v;
-  a.ee_v = nb.ee_v;
-  a.vf_w = nb.vf_w;
-  a.ee_w = nb.ee_w;
+  a.rec = nb.rec;
   return a;
 }
 
