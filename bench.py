@@ -60,7 +60,16 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=150)
+    ap.add_argument("--batch-cloths", type=int, default=512,
+                    help="cloths of the batched-vs-plain leg (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the batched and single-cloth paths shard by cloth "
               f"with no collectives, so --gpus {args.gpus} is not measured here.",
@@ -133,6 +142,8 @@ def main():
         stage_ms_after_run=stage_ms,
     )
     body.close()
+    out["batched"] = batched_leg(args.batch_cloths, args.batch_steps) if args.batch_cloths > 0 \
+        else None
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(configs.config_scene(1, args.size), args.cpu_budget)
         out["cpu_baseline"]["cpu_model"] = _cpu_model()
@@ -140,6 +151,61 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
+    """Cloth-steps/s of the 512-cloth batch: one batched body against one body per
+    cloth (each on its own stream, all launched before any is waited for)."""
+    from licenta_amd import configs
+    from licenta_amd.body import ClothBody
+
+    scenes = configs.batched_scene(2, count=ncloth)
+    nv = sum(len(s["xyz"]) for s in scenes)
+    nt = sum(len(s["tri"]) for s in scenes)
+    batch = ClothBody.batch(scenes)
+    batch.fixed_update(warmup)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.fixed_update(steps)
+    batch.sync()
+    tb = time.perf_counter() - t0
+    bcounts = batch.counts()
+    batch.close()
+    singles = [ClothBody.from_scene(s) for s in scenes]
+    for b in singles:
+        b.fixed_update(warmup)
+    for b in singles:
+        b.sync()
+    t0 = time.perf_counter()
+    for b in singles:
+        b.fixed_update(steps)
+    for b in singles:
+        b.sync()
+    tp = time.perf_counter() - t0
+    for b in singles:
+        b.close()
+    batched = ncloth * steps / tb
+    plain = ncloth * steps / tp
+    return dict(workload=f"{ncloth} cloths, sides U{{16..128}}, seed 2", cloths=ncloth,
+                vertices=nv, triangles=nt, steps=steps, unit="cloth-steps/s",
+                batched_cloth_steps_per_s=batched, plain_cloth_steps_per_s=plain,
+                batched_over_plain=batched / plain, batched_ms_per_step=tb * 1e3 / steps,
+                plain_ms_per_step=tp * 1e3 / steps,
+                last_substep=dict(pairs=bcounts["pairs"], vf=bcounts["vf"], ee=bcounts["ee"]))
+
+
+def calibrate_copy():
+    import torch
+
+    n = 64 * 1024 * 1024  # floats: 256 MiB
+    a = torch.empty(n, device="cuda", dtype=torch.float32).fill_(1.0)
+    c = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps(dict(calibration="device copy", copies=3, bytes_read_per_copy=4 * n,
+                          bytes_written_per_copy=4 * n)))
 
 
 def _cpu_model():

@@ -164,6 +164,23 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
               int32_t n_dist, const float* k_dist, const float* k_bend, cm_body** out);
 int cm_destroy(cm_body* body);
 
+/* A batch of independent cloths stepped as one body (no counterpart in the
+ * reference, whose bodies are stepped one at a time; each cloth follows
+ * cm_create's semantics on its own, Physics/CudaDeformableBody.cpp:550-627).
+ * Cloth c has cloth_n[c] vertices and cloth_t[c] triangles; xyz, pinned and
+ * inv_mass hold all vertices cloth after cloth, tri all triangles with cloth-local
+ * vertex ids; dist_pairs (NULL: mesh edges) holds cloth_ndist[c] cloth-local pairs
+ * per cloth; k_stretch / k_bend give one stiffness per cloth (NULL: params). Each
+ * cloth has its own LBVH (30-bit codes in the cloth's centroid box; cloths never
+ * collide with each other). Positions and buffers are read back for all vertices,
+ * cloth after cloth. Requires morton_bits 30 with morton_normalize, and no LRA or
+ * rigid-mode damping. */
+int cm_create_batch(const cm_params* params, int32_t ncloth, const int32_t* cloth_n,
+                    const int32_t* cloth_t, const int32_t* cloth_ndist, const float* xyz,
+                    const uint32_t* tri, const uint8_t* pinned, const float* inv_mass,
+                    const int32_t* dist_pairs, const float* k_stretch, const float* k_bend,
+                    cm_body** out);
+
 /* Runs `substeps` substeps on the body's stream (graph-replayed); does not sync. */
 int cm_step(cm_body* body, int32_t substeps);
 int cm_sync(cm_body* body);

@@ -53,7 +53,9 @@ assert CONTACT_DTYPE.itemsize == 48 == C.sizeof(CmContact)
 def make_params(d: dict) -> CmParams:
     p = CmParams()
     for name, typ in CmParams._fields_:
-        if name == "reserved":
+        if name == "reserved":  # [0..2]: VF / EE candidates and contacts per triangle
+            for i, x in enumerate(d.get("reserved", ())):
+                p.reserved[i] = int(x)
             continue
         val = d[name]
         if isinstance(val, (tuple, list, np.ndarray)):
@@ -90,6 +92,8 @@ def lib():
     L.cm_default_params.argtypes = [C.POINTER(CmParams)]
     L.cm_create.argtypes = [C.POINTER(CmParams), vp, i32, vp, i32, vp, vp, vp, i32, vp, vp,
                             C.POINTER(vp)]
+    L.cm_create_batch.argtypes = [C.POINTER(CmParams), i32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                  vp, C.POINTER(vp)]
     L.cm_destroy.argtypes = [vp]
     L.cm_step.argtypes = [vp, i32]
     L.cm_sync.argtypes = [vp]
@@ -127,7 +131,8 @@ def check(rc: int) -> None:
 
 # exported names declared in include/clothmi.h (kept in sync by tests/test_abi.py)
 EXPORTS = (
-    "cm_default_params", "cm_last_error", "cm_create", "cm_destroy", "cm_step", "cm_sync",
+    "cm_default_params", "cm_last_error", "cm_create", "cm_create_batch", "cm_destroy", "cm_step",
+    "cm_sync",
     "cm_read_positions", "cm_read_velocities", "cm_read_normals", "cm_set_sphere",
     "cm_set_state", "cm_get_counts", "cm_get_buffer", "cm_get_warnings", "cm_query_pairs",
     "cm_query_contacts", "cm_tri_update", "cm_lbvh_sort_temp_bytes", "cm_lbvh_sort",

@@ -58,6 +58,45 @@ class ClothBody:
         self.h = h
 
     @classmethod
+    def batch(cls, scenes: list, params: dict | None = None) -> "ClothBody":
+        """Independent cloths stepped together as one body (cm_create_batch). Every
+        cloth keeps its own LBVH and its k_stretch / k_bend from its scene; the other
+        parameters come from ``params`` (default: the first scene's). Vertices of
+        cloth c follow those of cloth c - 1 in every read-back."""
+        L = lib()
+        self = cls.__new__(cls)
+        self.params = dict(params or scenes[0]["params"])
+        self._p = make_params(self.params)
+        self.cloth_n = np.array([len(s["xyz"]) for s in scenes], dtype=np.int32)
+        self.cloth_t = np.array([len(s["tri"]) for s in scenes], dtype=np.int32)
+        self.xyz = np.ascontiguousarray(np.concatenate([s["xyz"] for s in scenes]), dtype=np.float32)
+        self.tri = np.ascontiguousarray(np.concatenate([s["tri"] for s in scenes]), dtype=np.uint32)
+        self.n, self.t = len(self.xyz), len(self.tri)
+        self.inv_mass = np.ascontiguousarray(np.concatenate([s["inv_mass"] for s in scenes]),
+                                             dtype=np.float32)
+        self.pinned = np.ascontiguousarray(np.concatenate(
+            [s["pinned"] if s.get("pinned") is not None else (s["inv_mass"] == 0) for s in scenes]),
+            dtype=np.uint8)
+        if all(s.get("dist_pairs") is not None for s in scenes):
+            self.cloth_nd = np.array([len(s["dist_pairs"]) for s in scenes], dtype=np.int32)
+            self.dist_pairs = np.ascontiguousarray(np.concatenate([s["dist_pairs"] for s in scenes]),
+                                                   dtype=np.int32)
+        else:
+            self.cloth_nd, self.dist_pairs = None, None
+        ks = np.array([s["params"].get("k_stretch", self.params["k_stretch"]) for s in scenes],
+                      dtype=np.float32)
+        kb = np.array([s["params"].get("k_bend", self.params["k_bend"]) for s in scenes],
+                      dtype=np.float32)
+        self.k_dist = self.k_bend = None
+        h = C.c_void_p()
+        check(L.cm_create_batch(C.byref(self._p), len(scenes), ptr(self.cloth_n), ptr(self.cloth_t),
+                                ptr(self.cloth_nd), ptr(self.xyz), ptr(self.tri), ptr(self.pinned),
+                                ptr(self.inv_mass), ptr(self.dist_pairs), ptr(ks), ptr(kb),
+                                C.byref(h)))
+        self.h = h
+        return self
+
+    @classmethod
     def from_scene(cls, scene: dict) -> "ClothBody":
         return cls(scene["xyz"], scene["tri"], scene["params"], scene.get("pinned"),
                    scene["inv_mass"], scene.get("dist_pairs"), scene.get("k_dist"),

@@ -33,6 +33,7 @@ using namespace cm;
 struct cm_body {
   cm_params prm;
   int n = 0, t = 0, S = 0, B = 0, F = 0, nslots = 0, cap = 0;
+  int key_bits = 30;  // bits of the Morton sort keys (a batch adds the cloth id)
   int dev = 0;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
@@ -399,14 +400,18 @@ int record_substep(cm_body* b, hipEvent_t* ev = nullptr) {
   const bool coll = (P.flags & CM_F_SELF_COLLISION) && b->t >= 2;
   mark(CM_STAGE_TRIANGLES);
   if (coll) {
-    rc = launch_tri_update(b->n, b->t, b->x, b->p, b->tri, P.thickness, P.morton_bits,
+    if (b->lb.ncloth)
+      rc = launch_tri_update_batch(b->lb, b->x, b->p, b->tri, P.thickness, b->keys, b->perm,
+                                   b->cen, s);
+    else
+      rc = launch_tri_update(b->n, b->t, b->x, b->p, b->tri, P.thickness, P.morton_bits,
                            P.morton_normalize, b->lb.amin, b->lb.amax, b->keys, b->perm, b->cen,
                            b->box, s);
     if (rc) return rc;
   }
   mark(CM_STAGE_SORT);
   if (coll) {
-    rc = radix_sort(b->keys, b->perm, b->t, P.morton_bits == 30 ? 30 : 63, b->sort_temp, s);
+    rc = radix_sort(b->keys, b->perm, b->t, b->key_bits, b->sort_temp, s);
     if (rc) return rc;
     rc = launch_gather_tris(b->t, b->perm, b->tri, b->tri_alt,
                             (P.ref_compat & CM_RC_STALE_LEAF_AABB) ? 0 : 1, b->lb.amin,
@@ -518,9 +523,23 @@ void cm_default_params(cm_params* p) {
   p->ref_compat = CM_RC_ALL;
 }
 
-int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32_t* tri,
-              int32_t t, const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
-              int32_t n_dist, const float* k_dist, const float* k_bend, cm_body** out) {
+}  // extern "C"
+
+namespace {
+
+// A batch of cloths packed into one body: cloth c owns vertices [vstart[c], vstart[c+1])
+// and triangles [tstart[c], tstart[c+1]); optional per-cloth stiffness.
+struct BatchSpec {
+  int ncloth = 0;
+  std::vector<int> vstart, tstart;
+  const float* k_stretch = nullptr;
+  const float* k_bend = nullptr;
+};
+
+int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint32_t* tri,
+                int32_t t, const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
+                int32_t n_dist, const float* k_dist, const float* k_bend, const BatchSpec* batch,
+                cm_body** out) {
   if (out) *out = nullptr;
   // vertex ids share a word with 3 flag bits in the contact records
   if (!params || !xyz || n < 1 || n >= (1 << 29) || !tri || t < 1 || !inv_mass || !out ||
@@ -560,6 +579,19 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   rc = build_topology(b->prm, xyz, n, tri, t, b->pinned_h.data(), dist_pairs, n_dist, k_dist,
                       k_bend, h);
   if (rc) return fail(rc);
+  if (batch && (batch->k_stretch || batch->k_bend)) {  // per-cloth stiffness
+    std::vector<int> cloth_of(n);
+    for (int c = 0; c < batch->ncloth; ++c)
+      for (int i = batch->vstart[c]; i < batch->vstart[c + 1]; ++i) cloth_of[i] = c;
+    const int I = b->prm.solver_iterations;
+    if (batch->k_stretch)
+      for (size_t i = 0; i < h.sid.size(); ++i)
+        h.s_l0kp[i].y = kp_of(batch->k_stretch[cloth_of[h.sid[i].x]], I);
+    if (batch->k_bend)
+      for (size_t i = 0; i < h.bid.size(); ++i)
+        h.b_kpphi0[i].x = kp_of(batch->k_bend[cloth_of[h.bid[i].x]], I);
+    build_tiles(n, h);  // tile entries carry the stiffness
+  }
   b->S = (int)h.sid.size();
   b->B = (int)h.bid.size();
   b->F = (int)h.lra_fixed.size();
@@ -610,6 +642,27 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   const int64_t pcap = (int64_t)t * b->cap;
   b->lb.t = t;
   b->lb.warn = b->warn;
+  b->key_bits = params->morton_bits == 30 ? 30 : 63;
+  if (batch) {
+    const int nc = batch->ncloth;
+    std::vector<int> slot_cloth(t), ccap(nc);
+    for (int c = 0; c < nc; ++c) {
+      for (int i = batch->tstart[c]; i < batch->tstart[c + 1]; ++i) slot_cloth[i] = c;
+      ccap[c] = std::min(params->pair_cap, batch->tstart[c + 1] - batch->tstart[c]);
+    }
+    int cbits = 0;
+    while ((1 << cbits) < nc) ++cbits;
+    b->key_bits = 30 + cbits;
+    b->lb.ncloth = nc;
+    int *d_tstart = nullptr, *d_slot = nullptr, *d_ccap = nullptr;
+    if ((rc = upload(b, &d_tstart, batch->tstart)) || (rc = upload(b, &d_slot, slot_cloth)) ||
+        (rc = upload(b, &d_ccap, ccap)) || (rc = dalloc(b, &b->lb.croot, (size_t)nc)) ||
+        (rc = dalloc(b, &b->lb.cbox, 6 * (size_t)nc)))
+      return fail(rc);
+    b->lb.tstart = d_tstart;
+    b->lb.slot_cloth = d_slot;
+    b->lb.ccap = d_ccap;
+  }
   if ((rc = dalloc(b, &b->keys, t)) || (rc = dalloc(b, &b->perm, t)) ||
       (rc = dalloc(b, (char**)&b->sort_temp, radix_sort_temp_bytes(t))) ||
       (rc = dalloc(b, &b->cen, t)) || (rc = dalloc(b, &b->box, 2 * (size_t)nblocks(t) + 2)) ||
@@ -685,6 +738,84 @@ int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32
   CM_HIP(hipDeviceSynchronize());
   *out = b;
   return CM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_create(const cm_params* params, const float* xyz, int32_t n, const uint32_t* tri,
+              int32_t t, const uint8_t* pinned, const float* inv_mass, const int32_t* dist_pairs,
+              int32_t n_dist, const float* k_dist, const float* k_bend, cm_body** out) {
+  return create_impl(params, xyz, n, tri, t, pinned, inv_mass, dist_pairs, n_dist, k_dist, k_bend,
+                     nullptr, out);
+}
+
+int cm_create_batch(const cm_params* params, int32_t ncloth, const int32_t* cloth_n,
+                    const int32_t* cloth_t, const int32_t* cloth_ndist, const float* xyz,
+                    const uint32_t* tri, const uint8_t* pinned, const float* inv_mass,
+                    const int32_t* dist_pairs, const float* k_stretch, const float* k_bend,
+                    cm_body** out) {
+  if (out) *out = nullptr;
+  if (!params || ncloth < 1 || !cloth_n || !cloth_t || !xyz || !tri || !inv_mass || !out ||
+      (dist_pairs && !cloth_ndist)) {
+    set_error("cm_create_batch: invalid argument");
+    return CM_ERR_ARG;
+  }
+  if (params->morton_bits != 30 || !params->morton_normalize ||
+      (params->flags & (CM_F_LRA | CM_F_DAMPING)) || ncloth > (1 << 20)) {
+    set_error("cm_create_batch: a batch needs 30-bit keys in each cloth's box, no LRA and no "
+              "rigid-mode damping, and at most 2^20 cloths");
+    return CM_ERR_ARG;
+  }
+  BatchSpec bs;
+  bs.ncloth = ncloth;
+  bs.k_stretch = k_stretch;
+  bs.k_bend = k_bend;
+  bs.vstart.assign(1, 0);
+  bs.tstart.assign(1, 0);
+  int64_t nd = 0;
+  for (int c = 0; c < ncloth; ++c) {
+    if (cloth_n[c] < 1 || cloth_t[c] < 1 || (cloth_ndist && cloth_ndist[c] < 0)) {
+      set_error("cm_create_batch: empty cloth");
+      return CM_ERR_ARG;
+    }
+    const int64_t nv = (int64_t)bs.vstart.back() + cloth_n[c];
+    const int64_t nt = (int64_t)bs.tstart.back() + cloth_t[c];
+    if (nv >= (1 << 29) || nt >= (1 << 30)) {
+      set_error("cm_create_batch: batch too large");
+      return CM_ERR_ARG;
+    }
+    bs.vstart.push_back((int)nv);
+    bs.tstart.push_back((int)nt);
+    if (cloth_ndist) nd += cloth_ndist[c];
+  }
+  const int n = bs.vstart.back(), t = bs.tstart.back();
+  // cloth-local ids -> body ids
+  std::vector<uint32_t> gtri(3 * (size_t)t);
+  for (int c = 0; c < ncloth; ++c)
+    for (int64_t i = 3 * (int64_t)bs.tstart[c]; i < 3 * (int64_t)bs.tstart[c + 1]; ++i) {
+      if (tri[i] >= (uint32_t)cloth_n[c]) {
+        set_error("cm_create_batch: triangle index out of range");
+        return CM_ERR_ARG;
+      }
+      gtri[i] = tri[i] + (uint32_t)bs.vstart[c];
+    }
+  std::vector<int32_t> gdist;
+  if (dist_pairs) {
+    gdist.resize(2 * (size_t)nd);
+    int64_t o = 0;
+    for (int c = 0; c < ncloth; ++c)
+      for (int64_t i = 0; i < 2 * (int64_t)cloth_ndist[c]; ++i, ++o) {
+        if (dist_pairs[o] < 0 || dist_pairs[o] >= cloth_n[c]) {
+          set_error("cm_create_batch: distance constraint vertex out of range");
+          return CM_ERR_ARG;
+        }
+        gdist[o] = dist_pairs[o] + bs.vstart[c];
+      }
+  }
+  return create_impl(params, xyz, n, gtri.data(), t, pinned, inv_mass,
+                     dist_pairs ? gdist.data() : nullptr, (int32_t)nd, nullptr, nullptr, &bs, out);
 }
 
 int cm_destroy(cm_body* b) {

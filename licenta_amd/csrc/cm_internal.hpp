@@ -77,6 +77,14 @@ struct LbvhBufs {
   int *trm = nullptr, *tdone = nullptr;
   float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
   uint32_t* warn = nullptr;
+  // batch of cloths (ncloth == 0: one cloth): cloth c owns triangle slots (and, after
+  // the sort, leaves) [tstart[c], tstart[c + 1]); its Morton keys carry c above bit 30
+  int ncloth = 0;
+  const int* tstart = nullptr;      // ncloth + 1
+  const int* slot_cloth = nullptr;  // T: cloth of each triangle slot / leaf
+  int* croot = nullptr;             // ncloth: internal node spanning the cloth (-1: none)
+  const int* ccap = nullptr;        // ncloth: per-leaf pair cap of the cloth
+  uint32_t* cbox = nullptr;         // 6 * ncloth: centroid box, order-preserving encoding
 };
 size_t lbvh_build_temp_bytes(int t);
 size_t carve_build(int t, void* d_temp, LbvhBufs* b);  // b == nullptr: size only
@@ -84,6 +92,10 @@ size_t lbvh_pairs_temp_bytes(int t, int cap);
 int launch_tri_update(int n, int t, const float4* x, const float4* p, const int4* tri, float th,
                       int bits, int normalize, float4* amin, float4* amax, uint64_t* keys,
                       uint32_t* vals, float4* centroid_tmp, float4* box_tmp, hipStream_t s);
+// batch: swept AABBs, per-cloth centroid boxes and keys (cloth << 30 | Morton30)
+int launch_tri_update_batch(const LbvhBufs& b, const float4* x, const float4* p, const int4* tri,
+                            float th, uint64_t* keys, uint32_t* vals, float4* centroid_tmp,
+                            hipStream_t s);
 int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStream_t s);
 int launch_refit(const LbvhBufs& b, hipStream_t s);
 int launch_pack_nodes(const LbvhBufs& b, hipStream_t s);

@@ -119,22 +119,99 @@ __global__ void __launch_bounds__(kBlock) k_keys30(int t, const float4* __restri
   keys[i] = morton30(xyz(centroid[i]), lo, hi);
 }
 
+// ---- per-cloth centroid boxes of a batch -------------------------------------------
+// Floats are mapped to unsigned integers of the same order, so the box is a pair of
+// integer atomic min/max per axis: exact, and independent of the order of arrival.
+__device__ __forceinline__ uint32_t ford(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float funord(uint32_t e) {
+  return __uint_as_float((e & 0x80000000u) ? (e & 0x7fffffffu) : ~e);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cloth_box_init(int ncloth, uint32_t* __restrict__ cbox) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < 6 * ncloth) cbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;  // min xyz, max xyz
+}
+
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u(uint32_t v) {
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// centroid box of each cloth; a wave whose triangles all belong to one cloth
+// reduces first and issues one atomic per bound
+__global__ void __launch_bounds__(kBlock) k_cloth_box(int t, const float4* __restrict__ cen,
+                                                      const int* __restrict__ slot_cloth,
+                                                      uint32_t* __restrict__ cbox) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < t;
+  const int c = valid ? slot_cloth[i] : -1;
+  const f3 p = valid ? xyz(cen[i]) : mk3(0, 0, 0);
+  uint32_t e[6];
+  e[0] = valid ? ford(p.x) : 0xffffffffu;
+  e[1] = valid ? ford(p.y) : 0xffffffffu;
+  e[2] = valid ? ford(p.z) : 0xffffffffu;
+  e[3] = valid ? ford(p.x) : 0u;
+  e[4] = valid ? ford(p.y) : 0u;
+  e[5] = valid ? ford(p.z) : 0u;
+  const int c0 = __shfl(c, 0, 64);
+  const int cmax = (int)wave_max_u((uint32_t)(c + 1)) - 1, cmin = (int)wave_min_u((uint32_t)(c + 1)) - 1;
+  const bool uniform = c0 >= 0 && cmin == c0 && cmax == c0;
+  if (uniform) {
+    for (int a = 0; a < 3; ++a) e[a] = wave_min_u(e[a]);
+    for (int a = 3; a < 6; ++a) e[a] = wave_max_u(e[a]);
+    if ((threadIdx.x & 63) == 0)
+      for (int a = 0; a < 6; ++a)
+        if (a < 3) atomicMin(&cbox[6 * c0 + a], e[a]); else atomicMax(&cbox[6 * c0 + a], e[a]);
+  } else if (valid) {
+    for (int a = 0; a < 6; ++a)
+      if (a < 3) atomicMin(&cbox[6 * c + a], e[a]); else atomicMax(&cbox[6 * c + a], e[a]);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_keys30_cloth(int t, const float4* __restrict__ cen,
+                                                         const int* __restrict__ slot_cloth,
+                                                         const uint32_t* __restrict__ cbox,
+                                                         uint64_t* __restrict__ keys) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= t) return;
+  const int c = slot_cloth[i];
+  const uint32_t* bx = cbox + 6 * c;
+  const f3 lo = mk3(funord(bx[0]), funord(bx[1]), funord(bx[2]));
+  const f3 hi = mk3(funord(bx[3]), funord(bx[4]), funord(bx[5]));
+  keys[i] = ((uint64_t)c << 30) | morton30(xyz(cen[i]), lo, hi);
+}
+
 // ---- Karras build -----------------------------------------------------------
-__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int i, int j, int maxj) {
+// Core/CudaUtils.cu:338-351. In a batch, equal keys belong to one cloth and the
+// index tie-break uses cloth-local indices, so each cloth's subtree is the tree its
+// own keys would build.
+__device__ __forceinline__ int delta(const uint64_t* __restrict__ k, int i, int j, int maxj,
+                                     const int* __restrict__ tstart) {
   if (j < 0 || j > maxj) return -1;
   const uint64_t x = k[i] ^ k[j];
-  return x ? clz64(x) : clz64((uint64_t)i ^ (uint64_t)j) + 64;
+  if (x) return clz64(x);
+  const int base = tstart ? tstart[(int)(k[i] >> 30)] : 0;
+  return clz64((uint64_t)(i - base) ^ (uint64_t)(j - base)) + 64;
 }
 __device__ __forceinline__ int sgn(int a) { return signbit((float)a) ? -1 : 1; }
 
 __global__ void __launch_bounds__(kBlock) k_build_init(int t, int* __restrict__ cnt,
                                                        int* __restrict__ hi,
-                                                       int* __restrict__ ntop) {
+                                                       int* __restrict__ ntop, int ncloth,
+                                                       int* __restrict__ croot) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 2 * t - 1) {
     cnt[i] = 0;
     hi[i] = -1;
   }
+  if (i < ncloth) croot[i] = -1;
   if (i == 0) *ntop = 0;  // deferred-node count of the refit
 }
 
@@ -142,35 +219,41 @@ __global__ void __launch_bounds__(kBlock) k_build(int t, const uint64_t* __restr
                                                   int midpoint, int* __restrict__ lefts,
                                                   int* __restrict__ rights, int* __restrict__ cnt,
                                                   int* __restrict__ hi, int2* __restrict__ ranges,
-                                                  uint32_t* __restrict__ warn) {
+                                                  uint32_t* __restrict__ warn,
+                                                  const int* __restrict__ tstart,
+                                                  int* __restrict__ croot) {
   const int id = blockIdx.x * kBlock + threadIdx.x;
   const int nin = t - 1;
   if (id >= nin) return;
   // range (Collision/CudaBVH.cu:190-212)
-  const int d = sgn(delta(k, id, id + 1, nin) - delta(k, id, id - 1, nin));
-  const int dmin = delta(k, id, id - d, nin);
+  const int d = sgn(delta(k, id, id + 1, nin, tstart) - delta(k, id, id - 1, nin, tstart));
+  const int dmin = delta(k, id, id - d, nin, tstart);
   int lmax = 2;
-  while (delta(k, id, id + lmax * d, nin) > dmin) lmax <<= 1;
+  while (delta(k, id, id + lmax * d, nin, tstart) > dmin) lmax <<= 1;
   int l = 0;
   for (int s = lmax / 2; s >= 1; s /= 2)
-    if (delta(k, id, id + (l + s) * d, nin) > dmin) l += s;
+    if (delta(k, id, id + (l + s) * d, nin, tstart) > dmin) l += s;
   const int j = id + l * d;
   const int first = min(id, j), last = max(id, j);
   ranges[id] = make_int2(first, last);
+  if (tstart) {  // the node spanning exactly one cloth is that cloth's root
+    const int c = (int)(k[first] >> 30);
+    if (first == tstart[c] && last == tstart[c + 1] - 1) croot[c] = id;
+  }
   // split (Collision/CudaBVH.cu:214-242)
   const uint64_t fc = k[first], lc = k[last];
   int split;
   if (midpoint && fc == lc) {
     split = (first + last) >> 1;
   } else {
-    const int com = midpoint ? clz64(fc ^ lc) : delta(k, first, last, nin);
+    const int com = midpoint ? clz64(fc ^ lc) : delta(k, first, last, nin, tstart);
     split = first;
     int step = last - first;
     do {
       step = (step + 1) >> 1;
       const int ns = split + step;
       if (ns < last) {
-        const int pref = midpoint ? clz64(fc ^ k[ns]) : delta(k, first, ns, nin);
+        const int pref = midpoint ? clz64(fc ^ k[ns]) : delta(k, first, ns, nin, tstart);
         if (pref > com) split = ns;
       }
     } while (step > 1);
@@ -435,7 +518,10 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __rest
                                                      const float4* __restrict__ amax, int cap,
                                                      int2* __restrict__ chunk,
                                                      int* __restrict__ counts,
-                                                     uint32_t* __restrict__ warn) {
+                                                     uint32_t* __restrict__ warn,
+                                                     const int* __restrict__ slot_cloth,
+                                                     const int* __restrict__ croot,
+                                                     const int* __restrict__ ccap) {
   __shared__ int sstk[kLdsStack][kBlock];
   // Workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous eighth
   // of the (Morton-ordered) leaves so its L2 holds the part of the tree it walks.
@@ -445,11 +531,20 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __rest
   const int me = nin + leaf;
   const f3 mn = xyz(amin[me]), mx = xyz(amax[me]);
   int2* out = chunk + (int64_t)leaf * cap;
+  int node = 0;  // root (a batch: the root of the leaf's cloth, with the cloth's cap)
+  if (croot) {
+    const int c = slot_cloth[leaf];
+    node = croot[c];
+    cap = ccap[c];
+    if (node < 0) {  // a one-triangle cloth has no pairs
+      counts[leaf] = 0;
+      return;
+    }
+  }
   int ncol = 0;
   uint32_t w = 0;
   int pstk[63 - kLdsStack];
   int top = 0;  // 63 usable entries, as the reference's int[64] with its -1 sentinel
-  int node = 0;
   const long long limit = 64ll * t + 4096;
   long long visits = 0;
   while (true) {
@@ -574,13 +669,31 @@ int launch_tri_update(int n, int t, const float4* x, const float4* p, const int4
   return CM_OK;
 }
 
+int launch_tri_update_batch(const LbvhBufs& b, const float4* x, const float4* p, const int4* tri,
+                            float th, uint64_t* keys, uint32_t* vals, float4* centroid_tmp,
+                            hipStream_t s) {
+  const int t = b.t;
+  if (t <= 0) return CM_OK;
+  hipLaunchKernelGGL(k_cloth_box_init, dim3(nblocks(6 * b.ncloth)), dim3(kBlock), 0, s, b.ncloth,
+                     b.cbox);
+  hipLaunchKernelGGL(k_tri_update, dim3(nblocks(t)), dim3(kBlock), 0, s, t, x, p, tri, th, 30,
+                     b.amin, b.amax, keys, vals, centroid_tmp, nullptr);
+  hipLaunchKernelGGL(k_cloth_box, dim3(nblocks(t)), dim3(kBlock), 0, s, t, centroid_tmp,
+                     b.slot_cloth, b.cbox);
+  hipLaunchKernelGGL(k_keys30_cloth, dim3(nblocks(t)), dim3(kBlock), 0, s, t, centroid_tmp,
+                     b.slot_cloth, b.cbox, keys);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
 int launch_build(const LbvhBufs& b, const uint64_t* keys, int midpoint, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return CM_OK;
-  hipLaunchKernelGGL(k_build_init, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
-                     b.claim_hi, b.ntop);
+  hipLaunchKernelGGL(k_build_init, dim3(nblocks(std::max(2 * t - 1, b.ncloth))), dim3(kBlock), 0,
+                     s, t, b.claim_cnt, b.claim_hi, b.ntop, b.ncloth, b.croot);
   hipLaunchKernelGGL(k_build, dim3(nblocks(t - 1)), dim3(kBlock), 0, s, t, keys, midpoint, b.lefts,
-                     b.rights, b.claim_cnt, b.claim_hi, b.ranges, b.warn);
+                     b.rights, b.claim_cnt, b.claim_hi, b.ranges, b.warn,
+                     b.ncloth ? b.tstart : nullptr, b.croot);
   hipLaunchKernelGGL(k_parents, dim3(nblocks(2 * t - 1)), dim3(kBlock), 0, s, t, b.claim_cnt,
                      b.claim_hi, b.parents);
   CM_LAUNCH_CHECK();
@@ -605,7 +718,8 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   const int t = b.t;
   if (t < 2) return fill_i32(d_npairs, 0, 1, s);
   hipLaunchKernelGGL(k_traverse, dim3(nblocks(t)), dim3(kBlock), 0, s, t, b.nodes, b.amin, b.amax,
-                     cap, chunk, counts, b.warn);
+                     cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
+                     b.ncloth ? b.croot : nullptr, b.ccap);
   CM_LAUNCH_CHECK();
   int rc = scan_exclusive(counts, offsets, t, nullptr, d_npairs, scan_temp, s);
   if (rc) return rc;

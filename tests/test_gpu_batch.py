@@ -1,0 +1,103 @@
+"""GPU parity of the batched body (cm_create_batch) against one body per cloth.
+
+A batch must step every cloth exactly as a body of its own would: the same
+positions bit for bit, and per cloth the same triangle-pair list (leaf ids shifted
+by the cloth's first triangle) and the same VF / EE contacts (vertex ids shifted by
+its first vertex). The single bodies are pinned to the oracle by test_gpu_body.py.
+"""
+import numpy as np
+import pytest
+
+from licenta_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes():
+    rng = np.random.default_rng(7)
+    scenes = configs.batched_scene(2, count=4)
+    # shrink the drawn sizes so the test stays quick
+    small = []
+    for s, (nx, ny) in zip(scenes, [(12, 20), (9, 9), (17, 6), (2, 2)]):
+        xyz = configs.flat_grid(nx, 0.01, rng, m=ny)
+        pinned = np.zeros(nx * ny, dtype=np.uint8)
+        pinned[0] = 1
+        inv = np.where(pinned == 1, 0.0, 1000.0).astype(np.float32)
+        small.append(configs._scene(dict(s["params"]), xyz, configs.grid_triangles(nx, ny, False),
+                                    inv, pinned, configs.grid_distance_pairs_fast(nx, ny),
+                                    name=f"b{nx}x{ny}"))
+    # a cloth folded onto itself (contacts) and one crushed into a 2 mm box with one
+    # pinned corner 1 m away, so the crushed triangles share Morton cells (tied codes)
+    n = 10
+    fold = configs.config_scene(0, n)
+    xyz = fold["xyz"].reshape(n, n, 3).copy()
+    h = n // 2
+    for i in range(h, n):
+        xyz[i, :, 0] = xyz[2 * h - 1 - i, :, 0] + 0.002
+        xyz[i, :, 1] = 0.003 + 0.0005 * (i - h)
+    fold["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    crush = configs.config_scene(0, 8)
+    crush["xyz"] = (0.5 + rng.uniform(-0.001, 0.001, size=crush["xyz"].shape)).astype(np.float32)
+    crush["xyz"][0] = (1.5, 1.5, 1.5)
+    for sc in (fold, crush):
+        sc["params"] = dict(small[0]["params"], k_stretch=0.8, k_bend=0.05)
+        small.append(sc)
+    return small
+
+
+def _compare(batch, singles, scenes):
+    x = batch.positions()
+    vo = to = 0
+    bp = batch.pairs()
+    bvf, bee = batch.contacts()
+    for s, body in zip(scenes, singles):
+        nv, nt = len(s["xyz"]), len(s["tri"])
+        assert np.array_equal(x[vo:vo + nv], body.positions()), s["name"]
+        sel = (bp[:, 0] >= to) & (bp[:, 0] < to + nt)
+        assert np.array_equal(bp[sel] - to, body.pairs()), s["name"]
+        for bc, sc in zip((bvf, bee), body.contacts()):
+            mine = bc[(bc["v"][:, 0] >= vo) & (bc["v"][:, 0] < vo + nv)]
+            assert np.array_equal(mine["v"] - vo, sc["v"]), s["name"]
+            assert np.array_equal(mine["w"], sc["w"]), s["name"]
+        vo += nv
+        to += nt
+
+
+def test_batch_matches_single_bodies():
+    from licenta_amd.body import ClothBody
+    scenes = _scenes()
+    params = dict(scenes[0]["params"], thickness=0.004)
+    singles = [ClothBody.from_scene(dict(s, params=dict(params, k_stretch=s["params"]["k_stretch"],
+                                                        k_bend=s["params"]["k_bend"])))
+               for s in scenes]
+    batch = ClothBody.batch(scenes, params)
+    # initial velocities: the folded half moves down onto the other half
+    vs = [np.zeros((len(s["xyz"]), 3), np.float32) for s in scenes]
+    vs[-2].reshape(10, 10, 3)[5:, :, 1] = -0.3
+    for body, s, v in zip(singles, scenes, vs):
+        body.set_state(s["xyz"], v)
+    batch.set_state(np.concatenate([s["xyz"] for s in scenes]), np.concatenate(vs))
+    total = 0
+    for step in range(4):
+        print(f"[batch] substep {step}", flush=True)
+        batch.fixed_update(1)
+        for body in singles:
+            body.fixed_update(1)
+        _compare(batch, singles, scenes)
+        c = batch.counts()
+        total += c["vf"] + c["ee"]
+    assert total > 0 and batch.counts()["pairs"] > 0
+    assert batch.warnings() & 4, "the crushed cloth should build a broken tree"
+    batch.close()
+    for body in singles:
+        body.close()
+
+
+def test_batch_rejects_unsupported_modes():
+    from licenta_amd.body import ClothBody
+    scenes = _scenes()[:2]
+    with pytest.raises(RuntimeError):
+        ClothBody.batch(scenes, dict(scenes[0]["params"], morton_bits=63))
+    with pytest.raises(RuntimeError):
+        ClothBody.batch(scenes, dict(scenes[0]["params"], flags=scenes[0]["params"]["flags"] |
+                                     configs.F_LRA))
