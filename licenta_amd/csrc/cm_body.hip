@@ -11,6 +11,7 @@
 // host once; everything per substep runs on the body's stream.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -562,6 +563,9 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
   b->prm = *params;
   b->n = n;
   b->t = t;
+  // CLOTHMI_GRAPH=0 launches every substep's kernels directly instead of replaying
+  // the captured substep graph (same kernels, same order).
+  if (const char* g = std::getenv("CLOTHMI_GRAPH")) b->use_graph = std::strcmp(g, "0") != 0;
   int rc = CM_OK;
   auto fail = [&](int code) {
     cm_destroy(b);
@@ -706,6 +710,7 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
   nb.d_nee = b->counters + 4;
   nb.d_nrec = b->counters + 5;
   nb.d_ntotal = b->counters + 6;
+  nb.d_nnear = b->counters + 7;
   nb.warn = b->warn;
   {
     // one scan temp for every scan of the substep, sized for the longest
@@ -714,10 +719,13 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
     CM_HIP(hipMemset(b->scan_temp, 0, scan_temp_bytes((int)smax)));
   }
   nb.scan_temp = b->scan_temp;
-  if ((rc = dalloc(b, &nb.mask, pcap)) || (rc = dalloc(b, &nb.cnt_vf, pcap)) ||
-      (rc = dalloc(b, &nb.cnt_ee, pcap)) || (rc = dalloc(b, &nb.cand_vf, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.cand_ee, nb.ee_cap)) || (rc = dalloc(b, &nb.flag_vf, nb.vf_cap)) ||
-      (rc = dalloc(b, &nb.flag_ee, nb.ee_cap)) ||
+  // tiles of 256 pairs or candidates (their counts, then the counts' prefixes)
+  const int64_t ntile = (std::max<int64_t>(pcap, nb.vf_cap + nb.ee_cap) + 255) / 256;
+  if ((rc = dalloc(b, &nb.mask, pcap)) || (rc = dalloc(b, &nb.tile_sum, ntile)) ||
+      (rc = dalloc(b, &nb.tile_pre, 2 * ntile)) || (rc = dalloc(b, &nb.cand_vf, nb.vf_cap)) ||
+      (rc = dalloc(b, &nb.cand_ee, nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.hit_flag, nb.vf_cap + nb.ee_cap)) ||
+      (rc = dalloc(b, &nb.near_ids, nb.vf_cap + nb.ee_cap)) ||
       (rc = dalloc(b, &nb.cplan, nb.vf_cap + nb.ee_cap)) ||
       (rc = dalloc(b, &nb.ccp, nb.vf_cap + nb.ee_cap)) ||
       (rc = dalloc(b, &nb.croot, 3 * (nb.vf_cap + nb.ee_cap))) ||

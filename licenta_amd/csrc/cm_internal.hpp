@@ -58,6 +58,12 @@ int scan_temp_clear(void* temp, int n_max, hipStream_t s);
 // *d_total (if non-null) receives the sum. out may alias in.
 int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_total, void* temp,
                    hipStream_t s);
+// Exclusive scan of per-tile sums (tiles of per_tile items; ceil(*d_items / per_tile)
+// of them, at most ntile_max; all ntile_max when d_items is null). packed: each sum
+// holds two counts below 2^16, scanned into pre[2i] (low) and pre[2i+1] (high);
+// else pre[i]. The totals, clamped to cap0 / cap1, go to *d_tot0 / *d_tot1.
+int scan_tiles(const int* d_items, int per_tile, int ntile_max, const int* sums, int* pre,
+               int packed, int64_t cap0, int64_t cap1, int* d_tot0, int* d_tot1, hipStream_t s);
 size_t radix_sort_temp_bytes(int n);
 int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s);
 int radix_sort_clear(void* temp, int n, hipStream_t s);  // before the first sort on a temp
@@ -124,10 +130,13 @@ struct NarrowBufs {
   const int2* pairs = nullptr;
   const int* d_npairs = nullptr;
   uint16_t* mask = nullptr;   // per pair
-  int *cnt_vf = nullptr, *cnt_ee = nullptr;  // per pair (scanned in place)
+  int* tile_sum = nullptr;  // per tile of 256 pairs / candidates: packed counts
+  int* tile_pre = nullptr;  // per tile: exclusive prefixes of the two counts
   int4 *cand_vf = nullptr, *cand_ee = nullptr;
   int *d_ncand_vf = nullptr, *d_ncand_ee = nullptr;
-  int *flag_vf = nullptr, *flag_ee = nullptr;  // per candidate (scanned in place)
+  uint8_t* hit_flag = nullptr;  // per candidate (VF then EE): near, then hit per near index
+  int* near_ids = nullptr;       // candidates passing the box-gap rejection, in order
+  int* d_nnear = nullptr;
   // CCD passes, per candidate (VF then EE, vf_cap + ee_cap): plan word, critical
   // points, and the secant root of each of its (up to three) intervals
   int* cplan = nullptr;

@@ -511,22 +511,24 @@ __global__ void __launch_bounds__(1024) k_refit_top(
 }
 
 // ---- traversal ----------------------------------------------------------------
-constexpr int kLdsStack = 24;  // per-thread stack entries kept in LDS, the rest private
+constexpr int kTravBlock = 128;  // 2 waves; the 63-entry stacks take 31.5 KiB of LDS
+constexpr int kStack = 63;       // usable entries, as the reference's int[64] with its -1 sentinel
 
-__global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __restrict__ nodes,
-                                                     const float4* __restrict__ amin,
-                                                     const float4* __restrict__ amax, int cap,
-                                                     int2* __restrict__ chunk,
-                                                     int* __restrict__ counts,
-                                                     uint32_t* __restrict__ warn,
-                                                     const int* __restrict__ slot_cloth,
-                                                     const int* __restrict__ croot,
-                                                     const int* __restrict__ ccap) {
-  __shared__ int sstk[kLdsStack][kBlock];
-  // Workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous eighth
-  // of the (Morton-ordered) leaves so its L2 holds the part of the tree it walks.
-  const int leaf = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (leaf >= t) return;
+__device__ __forceinline__ void put_pair(int2* p, int a, int b) {
+  // streamed past the caches so the node records stay resident in L2
+  __builtin_nontemporal_store((long long)(uint32_t)a | ((long long)b << 32), (long long*)p);
+}
+
+// DFS of one leaf (Collision/CudaBVH.cu:23-113): its pairs in the reference's order
+// into out, at most cap of them; returns their number.
+__device__ __forceinline__ int leaf_pairs(int t, int leaf, const float4* __restrict__ nodes,
+                                          const float4* __restrict__ amin,
+                                          const float4* __restrict__ amax, int cap,
+                                          int2* __restrict__ chunk, uint32_t* __restrict__ warn,
+                                          const int* __restrict__ slot_cloth,
+                                          const int* __restrict__ croot,
+                                          const int* __restrict__ ccap,
+                                          int (*sstk)[kTravBlock]) {
   const int nin = t - 1;
   const int me = nin + leaf;
   const f3 mn = xyz(amin[me]), mx = xyz(amax[me]);
@@ -536,15 +538,11 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __rest
     const int c = slot_cloth[leaf];
     node = croot[c];
     cap = ccap[c];
-    if (node < 0) {  // a one-triangle cloth has no pairs
-      counts[leaf] = 0;
-      return;
-    }
+    if (node < 0) return 0;  // a one-triangle cloth has no pairs
   }
   int ncol = 0;
   uint32_t w = 0;
-  int pstk[63 - kLdsStack];
-  int top = 0;  // 63 usable entries, as the reference's int[64] with its -1 sentinel
+  int top = 0;
   const long long limit = 64ll * t + 4096;
   long long visits = 0;
   while (true) {
@@ -552,42 +550,78 @@ __global__ void __launch_bounds__(kBlock) k_traverse(int t, const float4* __rest
       w |= CM_WARN_TRAVERSAL_LIMIT;
       break;
     }
+    // One 64-byte record per node: children, rightmost leaves, both child boxes.
+    // Both tests use & so that all four loads are issued together.
     const float4* nd = nodes + 4 * (int64_t)node;
     const float4 r0 = nd[0], r1 = nd[1], r2 = nd[2], r3 = nd[3];
     const int l = __float_as_int(r0.x), r = __float_as_int(r0.y);
     const int rml = __float_as_int(r0.z), rmr = __float_as_int(r0.w);
-    const bool ol = rml > me && overlap(mn, mx, mk3(r1.x, r1.y, r1.z), mk3(r1.w, r2.x, r2.y));
-    const bool orr = rmr > me && overlap(mn, mx, mk3(r2.z, r2.w, r3.x), mk3(r3.y, r3.z, r3.w));
-    // Past the cap the reference keeps walking but drops every further pair, so
-    // stopping at the first dropped pair yields the same list.
-    if (ol && l >= nin) {
-      if (ncol == cap) { w |= CM_WARN_PAIR_CAP; break; }
-      out[ncol++] = make_int2(leaf, l - nin);
-    }
-    if (orr && r >= nin) {
-      if (ncol == cap) { w |= CM_WARN_PAIR_CAP; break; }
-      out[ncol++] = make_int2(leaf, r - nin);
-    }
-    const bool tl = ol && l < nin, tr = orr && r < nin;
-    if (!tl && !tr) {
-      if (top == 0) break;
-      --top;
-      node = top < kLdsStack ? sstk[top][threadIdx.x] : pstk[top - kLdsStack];
-    } else {
-      node = tl ? l : r;
-      if (tl && tr) {
-        if (top < 63) {
-          if (top < kLdsStack) sstk[top][threadIdx.x] = r; else pstk[top - kLdsStack] = r;
-          ++top;
-        } else {
-          w |= CM_WARN_STACK_OVERFLOW;
-        }
+    const bool ol = (rml > me) & overlap(mn, mx, mk3(r1.x, r1.y, r1.z), mk3(r1.w, r2.x, r2.y));
+    const bool orr = (rmr > me) & overlap(mn, mx, mk3(r2.z, r2.w, r3.x), mk3(r3.y, r3.z, r3.w));
+    const bool el = ol & (l >= nin), er = orr & (r >= nin);
+    if (el | er) {
+      // Past the cap the reference keeps walking but drops every further pair, so
+      // stopping at the first dropped pair yields the same list.
+      if (ncol + (int)el + (int)er > cap) {
+        if (el && ncol < cap) put_pair(out + ncol++, leaf, l - nin);
+        else if (er && ncol < cap) put_pair(out + ncol++, leaf, r - nin);
+        w |= CM_WARN_PAIR_CAP;
+        break;
       }
+      if (el) put_pair(out + ncol++, leaf, l - nin);
+      if (er) put_pair(out + ncol++, leaf, r - nin);
+    }
+    const bool tl = ol & (l < nin), tr = orr & (r < nin);
+    if (tl | tr) {
+      node = tl ? l : r;
+      if (tl & tr) {
+        if (top < kStack) sstk[top++][threadIdx.x] = r;
+        else w |= CM_WARN_STACK_OVERFLOW;  // the push is dropped, as in the reference
+      }
+    } else {
+      if (top == 0) break;
+      node = sstk[--top][threadIdx.x];
     }
   }
-  counts[leaf] = ncol;
   if (w) atomicOr(warn, w);
+  return ncol;
 }
+
+__global__ void __launch_bounds__(kTravBlock) k_traverse(int t, const float4* __restrict__ nodes,
+                                                         const float4* __restrict__ amin,
+                                                         const float4* __restrict__ amax, int cap,
+                                                         int2* __restrict__ chunk,
+                                                         int* __restrict__ counts,
+                                                         uint32_t* __restrict__ warn,
+                                                         const int* __restrict__ slot_cloth,
+                                                         const int* __restrict__ croot,
+                                                         const int* __restrict__ ccap,
+                                                         int* __restrict__ tile_sum) {
+  // Entry k of every lane's stack is one row, so a wave's pushes and pops at equal
+  // depths hit 64 consecutive banks.
+  __shared__ int sstk[kStack][kTravBlock];
+  // Workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous eighth
+  // of the (Morton-ordered) leaves so its L2 holds the part of the tree it walks.
+  const int tile = xcd_block(blockIdx.x, gridDim.x);
+  const int leaf = tile * kTravBlock + threadIdx.x;
+  const int ncol = leaf < t ? leaf_pairs(t, leaf, nodes, amin, amax, cap, chunk, warn, slot_cloth,
+                                         croot, ccap, sstk)
+                            : 0;
+  if (leaf < t) counts[leaf] = ncol;
+  // pairs of the tile (its leaves' counts follow in the compaction)
+  __shared__ int wsum[kTravBlock / 64];
+  int v = ncol;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kTravBlock / 64; ++w) tot += wsum[w];
+    tile_sum[tile] = tot;
+  }
+}
+
 
 // packed node records from separate arrays (unit-level pair query)
 __global__ void __launch_bounds__(kBlock) k_pack_nodes(int t, const int* __restrict__ lefts,
@@ -604,16 +638,23 @@ __global__ void __launch_bounds__(kBlock) k_pack_nodes(int t, const int* __restr
              xyz(amax[r]));
 }
 
-// one wave per leaf copies its chunk to the compacted list
+// One wave per leaf copies its chunk to the compacted list. The leaf's offset is its
+// tile's prefix plus the counts of the tile's earlier leaves (at most 127, summed by
+// the wave).
 __global__ void __launch_bounds__(kBlock) k_compact_pairs(int t, int cap,
                                                           const int2* __restrict__ chunk,
                                                           const int* __restrict__ counts,
-                                                          const int* __restrict__ offsets,
+                                                          const int* __restrict__ tile_pre,
                                                           int2* __restrict__ pairs) {
   const int leaf = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (leaf >= t) return;
-  const int c = counts[leaf], o = offsets[leaf];
+  const int first = leaf / kTravBlock * kTravBlock;
+  int before = 0;
+  for (int j = first + lane; j < leaf; j += 64) before += counts[j];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) before += __shfl_xor(before, o, 64);
+  const int c = counts[leaf], o = tile_pre[leaf / kTravBlock] + before;
   const int2* src = chunk + (int64_t)leaf * cap;
   for (int k = lane; k < c; k += 64) pairs[o + k] = src[k];
 }
@@ -717,14 +758,18 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
                  int* d_npairs, void* scan_temp, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return fill_i32(d_npairs, 0, 1, s);
-  hipLaunchKernelGGL(k_traverse, dim3(nblocks(t)), dim3(kBlock), 0, s, t, b.nodes, b.amin, b.amax,
+  // offsets[0, nt): pair count of each tile of kTravBlock leaves; offsets[nt, 2nt): the
+  // tiles' exclusive prefixes (2 * nt <= t)
+  const int nt = (t + kTravBlock - 1) / kTravBlock;
+  hipLaunchKernelGGL(k_traverse, dim3(nt), dim3(kTravBlock), 0, s, t, b.nodes, b.amin, b.amax,
                      cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
-                     b.ncloth ? b.croot : nullptr, b.ccap);
+                     b.ncloth ? b.croot : nullptr, b.ccap, offsets);
   CM_LAUNCH_CHECK();
-  int rc = scan_exclusive(counts, offsets, t, nullptr, d_npairs, scan_temp, s);
+  int rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
+                      nullptr, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_compact_pairs, dim3(nblocks((int64_t)t * 64)), dim3(kBlock), 0, s, t, cap,
-                     chunk, counts, offsets, pairs);
+                     chunk, counts, offsets + nt, pairs);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

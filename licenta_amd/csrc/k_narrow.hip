@@ -8,7 +8,7 @@
 //        vertex and edge swept AABBs are recomputed from p and x on the fly
 //        (same values as Collision/FeatureList.cu:134-162), which removes the
 //        reference's N x N edge map.
-//  k_ccd <- CudaPrimitiveTests::CCDTriangleTests (Collision/CudaPrimitiveTests.cu:109-238)
+//  k_ccd_* <- CudaPrimitiveTests::CCDTriangleTests (Collision/CudaPrimitiveTests.cu:109-238)
 //        with Physics::FindCoplanarityTimes (Physics/CudaCubicSolver.cu:5-153)
 //  compaction keeps candidate order, like CubWrap::SelectFlagged (Core/CubWrappers.cuh:24-50)
 //  dynamic CSR: records (contact, role) grouped by vertex in record order, the order
@@ -21,6 +21,45 @@ namespace cm {
 namespace {
 
 __device__ __forceinline__ int bit(int f, int b) { return (f >> b) & 1; }
+
+// Items are processed in tiles, one tile of 64 * R consecutive items per wave: lane
+// l takes items base + l + 64 * j, j < R, so every load is coalesced and a tile needs
+// no workgroup barrier. The pass over all candidates uses R = 8 (fewer tile sums to
+// scan); the passes over pairs and over the short list of near candidates use R = 4
+// (more waves share the work).
+constexpr int kBig = 8, kSmall = 4;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Offsets of a tile's items within the tile, in item order: v[j] is the (packed)
+// count of item base + lane + 64 * j; off[j] receives the sum of the counts of all
+// earlier items of the tile.
+template <int R>
+__device__ __forceinline__ void tile_offsets(const int (&v)[R], int (&off)[R]) {
+  const int lane = threadIdx.x & 63;
+  int inc[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) inc[j] = v[j];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int u = __shfl_up(inc[j], o, 64);
+      if (lane >= o) inc[j] += u;
+    }
+  }
+  int carry = 0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    off[j] = carry + inc[j] - v[j];
+    carry += __shfl(inc[j], 63, 64);
+  }
+}
 
 __device__ __forceinline__ void vert_aabb(const float4* __restrict__ p,
                                           const float4* __restrict__ x, int i, float th, f3& mn,
@@ -37,74 +76,114 @@ __constant__ int kEE2[9][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 2}, {0, 2}, {0, 2}, {
 __constant__ int kEEB1[9] = {2, 0, 1, 2, 0, 1, 2, 0, 1};
 __constant__ int kEEB2[9] = {2, 2, 2, 0, 0, 0, 1, 1, 1};
 
+// Feature mask of one pair (bits 0-5: VF slots, 6-14: EE slots).
+__device__ __forceinline__ unsigned pair_mask(int2 pr, const int4* __restrict__ tri,
+                                              const float4* __restrict__ x,
+                                              const float4* __restrict__ p, float th, int ee_on) {
+  const int4 t1 = tri[pr.x], t2 = tri[pr.y];
+  const int v1[3] = {t1.x, t1.y, t1.z}, v2[3] = {t2.x, t2.y, t2.z};
+  f3 mn1[3], mx1[3], mn2[3], mx2[3];
+  for (int a = 0; a < 3; ++a) {
+    vert_aabb(p, x, v1[a], th, mn1[a], mx1[a]);
+    vert_aabb(p, x, v2[a], th, mn2[a], mx2[a]);
+  }
+  unsigned m = 0;
+  // VF: vertices of tri1 against tri2, then vertices of tri2 against tri1
+  const f3 tmn2 = min3(mn2[0], mn2[1], mn2[2]), tmx2 = max3(mx2[0], mx2[1], mx2[2]);
+  const f3 tmn1 = min3(mn1[0], mn1[1], mn1[2]), tmx1 = max3(mx1[0], mx1[1], mx1[2]);
+  for (int s = 0; s < 3; ++s) {
+    if (bit(t1.w, 5 - s) && overlap(mn1[s], mx1[s], tmn2, tmx2)) m |= 1u << s;
+    if (bit(t2.w, 5 - s) && overlap(mn2[s], mx2[s], tmn1, tmx1)) m |= 1u << (3 + s);
+  }
+  if (ee_on) {
+    for (int s = 0; s < 9; ++s) {
+      if (!(bit(t1.w, kEEB1[s]) & bit(t2.w, kEEB2[s]))) continue;
+      const int a0 = kEE1[s][0], a1 = kEE1[s][1], b0 = kEE2[s][0], b1 = kEE2[s][1];
+      const f3 amn = vmin(mn1[a0], mn1[a1]), amx = vmax(mx1[a0], mx1[a1]);
+      const f3 bmn = vmin(mn2[b0], mn2[b1]), bmx = vmax(mx2[b0], mx2[b1]);
+      if (overlap(amn, amx, bmn, bmx)) m |= 1u << (6 + s);
+    }
+  }
+  return m;
+}
+
+// VF and EE candidate counts of a mask, packed low / high
+__device__ __forceinline__ int mask_counts(unsigned m) {
+  return __popc(m & 63u) | (__popc(m >> 6) << 16);
+}
+
+// pass 1: feature mask per pair, candidate counts per tile of pairs
 __global__ void __launch_bounds__(kBlock) k_cand_count(
     const int2* __restrict__ pairs, const int* __restrict__ d_npairs, int64_t pair_cap,
     const int4* __restrict__ tri, const float4* __restrict__ x, const float4* __restrict__ p,
-    float th, int ee_on, uint16_t* __restrict__ mask, int* __restrict__ cnt_vf,
-    int* __restrict__ cnt_ee) {
+    float th, int ee_on, uint16_t* __restrict__ mask, int* __restrict__ tile_sum) {
   const int np = (int)min((int64_t)*d_npairs, pair_cap);
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < np; k += gridDim.x * kBlock) {
-    const int2 pr = pairs[k];
-    const int4 t1 = tri[pr.x], t2 = tri[pr.y];
-    const int v1[3] = {t1.x, t1.y, t1.z}, v2[3] = {t2.x, t2.y, t2.z};
-    f3 mn1[3], mx1[3], mn2[3], mx2[3];
-    for (int a = 0; a < 3; ++a) {
-      vert_aabb(p, x, v1[a], th, mn1[a], mx1[a]);
-      vert_aabb(p, x, v2[a], th, mn2[a], mx2[a]);
-    }
-    unsigned m = 0;
-    // VF: vertices of tri1 against tri2, then vertices of tri2 against tri1
-    const f3 tmn2 = min3(mn2[0], mn2[1], mn2[2]), tmx2 = max3(mx2[0], mx2[1], mx2[2]);
-    const f3 tmn1 = min3(mn1[0], mn1[1], mn1[2]), tmx1 = max3(mx1[0], mx1[1], mx1[2]);
-    for (int s = 0; s < 3; ++s) {
-      if (bit(t1.w, 5 - s) && overlap(mn1[s], mx1[s], tmn2, tmx2)) m |= 1u << s;
-      if (bit(t2.w, 5 - s) && overlap(mn2[s], mx2[s], tmn1, tmx1)) m |= 1u << (3 + s);
-    }
-    if (ee_on) {
-      for (int s = 0; s < 9; ++s) {
-        if (!(bit(t1.w, kEEB1[s]) & bit(t2.w, kEEB2[s]))) continue;
-        const int a0 = kEE1[s][0], a1 = kEE1[s][1], b0 = kEE2[s][0], b1 = kEE2[s][1];
-        const f3 amn = vmin(mn1[a0], mn1[a1]), amx = vmax(mx1[a0], mx1[a1]);
-        const f3 bmn = vmin(mn2[b0], mn2[b1]), bmx = vmax(mx2[b0], mx2[b1]);
-        if (overlap(amn, amx, bmn, bmx)) m |= 1u << (6 + s);
+  const int ntiles = (np + (64 * kSmall) - 1) / (64 * kSmall);
+  const int lane = threadIdx.x & 63;
+  for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * kWavesPerBlock) {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+      const int k = tile * (64 * kSmall) + j * 64 + lane;
+      if (k < np) {
+        const unsigned m = pair_mask(pairs[k], tri, x, p, th, ee_on);
+        mask[k] = (uint16_t)m;
+        c += mask_counts(m);
       }
     }
-    mask[k] = (uint16_t)m;
-    cnt_vf[k] = __popc(m & 63u);
-    cnt_ee[k] = __popc(m >> 6);
+    c = wave_sum(c);
+    if (lane == 0) tile_sum[tile] = c;
   }
 }
 
+// pass 2 (after the scan of the tile counts): candidates at their ordered positions
 __global__ void __launch_bounds__(kBlock) k_cand_write(
     const int2* __restrict__ pairs, const int* __restrict__ d_npairs, int64_t pair_cap,
     const int4* __restrict__ tri, const uint16_t* __restrict__ mask,
-    const int* __restrict__ off_vf, const int* __restrict__ off_ee, int4* __restrict__ cand_vf,
-    int4* __restrict__ cand_ee, int64_t vf_cap, int64_t ee_cap, uint32_t* __restrict__ warn) {
+    const int* __restrict__ tile_pre, int4* __restrict__ cand_vf, int4* __restrict__ cand_ee,
+    int64_t vf_cap, int64_t ee_cap, uint32_t* __restrict__ warn) {
   const int np = (int)min((int64_t)*d_npairs, pair_cap);
-  for (int k = blockIdx.x * kBlock + threadIdx.x; k < np; k += gridDim.x * kBlock) {
-    const unsigned m = mask[k];
-    if (!m) continue;
-    const int2 pr = pairs[k];
-    const int4 t1 = tri[pr.x], t2 = tri[pr.y];
-    const int v1[3] = {t1.x, t1.y, t1.z}, v2[3] = {t2.x, t2.y, t2.z};
-    int o = off_vf[k];
-    bool over = false;
-    for (int s = 0; s < 6; ++s) {
-      if (!(m & (1u << s))) continue;
-      const int4 q = s < 3 ? make_int4(v1[s], t2.x, t2.y, t2.z)
-                           : make_int4(v2[s - 3], t1.x, t1.y, t1.z);
-      if (o < vf_cap) cand_vf[o] = q; else over = true;
-      ++o;
+  const int ntiles = (np + (64 * kSmall) - 1) / (64 * kSmall);
+  const int lane = threadIdx.x & 63;
+  bool over = false;
+  for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * kWavesPerBlock) {
+    unsigned m[kSmall];
+    int c[kSmall], off[kSmall];
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+      const int k = tile * (64 * kSmall) + j * 64 + lane;
+      m[j] = k < np ? mask[k] : 0u;
+      c[j] = mask_counts(m[j]);
     }
-    o = off_ee[k];
-    for (int s = 0; s < 9; ++s) {
-      if (!(m & (1u << (6 + s)))) continue;
-      const int4 q = make_int4(v1[kEE1[s][0]], v1[kEE1[s][1]], v2[kEE2[s][0]], v2[kEE2[s][1]]);
-      if (o < ee_cap) cand_ee[o] = q; else over = true;
-      ++o;
+    tile_offsets<kSmall>(c, off);
+    const int pvf = tile_pre[2 * tile], pee = tile_pre[2 * tile + 1];
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+      if (!m[j]) continue;
+      const int k = tile * (64 * kSmall) + j * 64 + lane;
+      const int2 pr = pairs[k];
+      const int4 t1 = tri[pr.x], t2 = tri[pr.y];
+      const int v1[3] = {t1.x, t1.y, t1.z}, v2[3] = {t2.x, t2.y, t2.z};
+      int64_t o = (int64_t)pvf + (off[j] & 0xffff);
+      for (int s = 0; s < 6; ++s) {
+        if (!(m[j] & (1u << s))) continue;
+        const int4 q = s < 3 ? make_int4(v1[s], t2.x, t2.y, t2.z)
+                             : make_int4(v2[s - 3], t1.x, t1.y, t1.z);
+        if (o < vf_cap) cand_vf[o] = q; else over = true;
+        ++o;
+      }
+      o = (int64_t)pee + (off[j] >> 16);
+      for (int s = 0; s < 9; ++s) {
+        if (!(m[j] & (1u << (6 + s)))) continue;
+        const int4 q = make_int4(v1[kEE1[s][0]], v1[kEE1[s][1]], v2[kEE2[s][0]], v2[kEE2[s][1]]);
+        if (o < ee_cap) cand_ee[o] = q; else over = true;
+        ++o;
+      }
     }
-    if (over) atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
   }
+  if (over) atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
 }
 
 // ---- CCD (FP32, same arithmetic as the oracle) -------------------------------
@@ -295,7 +374,11 @@ __device__ __forceinline__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, floa
 //                 each lane taking the next candidate when its searches end, so a
 //                 long search does not idle the others; root of interval i of
 //                 candidate id in roots[3 * id + i]
-//   k_ccd_test    per candidate: the roots in order, cumulative advance, tests
+//   k_ccd_near    per candidate: the roots in order, cumulative advance, box-gap
+//                 rejection only; near flag and near count per tile
+//   k_near_list   ids of the near candidates in order
+//   k_ccd_test    per near candidate: the full tests; hit flag and hit counts
+//   k_ccd_write   hits at their ordered positions in the VF / EE contact lists
 // Plan word: bits 0-1 cc, bits 2-4 root at the start of interval i, bits 5-7 sign
 // change on interval i, bit 8 nothing to test (degenerate candidate).
 constexpr int kPlanDead = 1 << 8;
@@ -504,57 +587,179 @@ __device__ __forceinline__ bool ccd_tests(bool isvf, int4 c, int pl, float2 cp, 
   return false;
 }
 
-// pass 4: hit flags (VF then EE candidates)
-__global__ void __launch_bounds__(kBlock) k_ccd_test(
+// The candidate's roots in order with the same cumulative advance as ccd_tests, but
+// only the box-gap rejection at each root: false when every root is rejected, so the
+// full tests cannot hit. Most candidates end here, and the full tests run on the few
+// others without the divergence of the whole set.
+__device__ __forceinline__ bool ccd_near(bool isvf, int4 c, int pl, float2 cp, int id,
+                                         const float* __restrict__ roots,
+                                         const float4* __restrict__ x,
+                                         const float4* __restrict__ v, float th, float dt) {
+  if (pl & kPlanDead) return false;
+  const int4 q4 = cubic_ids(isvf, c);
+  f3 p1 = xyz(x[q4.x]), p2 = xyz(x[q4.y]), p3 = xyz(x[q4.z]), p4 = xyz(x[q4.w]);
+  const f3 u1 = xyz(v[q4.x]), u2 = xyz(v[q4.y]), u3 = xyz(v[q4.z]), u4 = xyz(v[q4.w]);
+  auto near_at = [&](float root) {
+    const float t = root * dt;
+    if (isvf) {
+      p4 = add(p4, mul(u4, t));
+      p1 = add(p1, mul(u1, t));
+      p2 = add(p2, mul(u2, t));
+      p3 = add(p3, mul(u3, t));
+      return !vf_far(p4, p1, p2, p3, th);
+    }
+    p1 = add(p1, mul(u1, t));
+    p2 = add(p2, mul(u2, t));
+    p3 = add(p3, mul(u3, t));
+    p4 = add(p4, mul(u4, t));
+    return !ee_far(p1, p2, p3, p4, th);
+  };
+  const int cc = pl & 3;
+  float ea = 0.f;
+  for (int i = 0; i <= cc; ++i) {
+    if ((pl & (1 << (2 + i))) && near_at(ea)) return true;
+    if (pl & (1 << (5 + i))) {
+      const float r = roots[3 * (int64_t)id + i];
+      if (r != kNoRoot && near_at(r)) return true;
+    }
+    ea = cubic_end(cp.x, cp.y, cc, i);
+  }
+  return false;
+}
+
+// pass 4: near flag per candidate (VF then EE ids), near counts per tile of ids
+__global__ void __launch_bounds__(kBlock) k_ccd_near(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
     const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
     const int* __restrict__ plan, const float2* __restrict__ cps,
     const float* __restrict__ roots, const float4* __restrict__ x, const float4* __restrict__ v,
-    float th, float dt, int* __restrict__ flag_vf, int* __restrict__ flag_ee) {
+    float th, float dt, uint8_t* __restrict__ near_flag, int* __restrict__ tile_sum) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total; id += gridDim.x * kBlock) {
-    bool isvf;
-    const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
-    Hit h;
-    const bool hit = ccd_tests(isvf, c, plan[id], cps[id], id, roots, x, v, th, dt, h);
-    if (isvf) flag_vf[id] = hit; else flag_ee[id - nvf] = hit;
+  const int ntiles = (total + (64 * kBig) - 1) / (64 * kBig);
+  const int lane = threadIdx.x & 63;
+  for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * kWavesPerBlock) {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kBig; ++j) {
+      const int id = tile * (64 * kBig) + j * 64 + lane;
+      if (id < total) {
+        bool isvf;
+        const int4 q = cand_of(id, nvf, cand_vf, cand_ee, isvf);
+        const bool nr = ccd_near(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt);
+        near_flag[id] = (uint8_t)nr;
+        c += nr;
+      }
+    }
+    c = wave_sum(c);
+    if (lane == 0) tile_sum[tile] = c;
   }
 }
 
-// pass 5: flags were scanned in place into offsets (total in *d_n); each hit is
-// recomputed and written at its compacted position, keeping candidate order.
+// pass 5: the near candidates' ids in order
+__global__ void __launch_bounds__(kBlock) k_near_list(const int* __restrict__ d_ntotal,
+                                                      const uint8_t* __restrict__ near_flag,
+                                                      const int* __restrict__ tile_pre,
+                                                      int* __restrict__ near_ids) {
+  const int total = *d_ntotal;
+  const int ntiles = (total + (64 * kBig) - 1) / (64 * kBig);
+  const int lane = threadIdx.x & 63;
+  for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * kWavesPerBlock) {
+    int c[kBig], off[kBig];
+#pragma unroll
+    for (int j = 0; j < kBig; ++j) {
+      const int id = tile * (64 * kBig) + j * 64 + lane;
+      c[j] = id < total ? near_flag[id] : 0;
+    }
+    tile_offsets<kBig>(c, off);
+    const int pre = tile_pre[tile];
+#pragma unroll
+    for (int j = 0; j < kBig; ++j)
+      if (c[j]) near_ids[pre + off[j]] = tile * (64 * kBig) + j * 64 + lane;
+  }
+}
+
+// pass 6: full tests of the near candidates; hit flag per near index, hit counts
+// (VF low, EE high) per tile of near indices
+__global__ void __launch_bounds__(kBlock) k_ccd_test(
+    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
+    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_nnear,
+    const int* __restrict__ near_ids, const int* __restrict__ plan,
+    const float2* __restrict__ cps, const float* __restrict__ roots,
+    const float4* __restrict__ x, const float4* __restrict__ v, float th, float dt,
+    uint8_t* __restrict__ hit_flag, int* __restrict__ tile_sum) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nn = *d_nnear;
+  const int ntiles = (nn + (64 * kSmall) - 1) / (64 * kSmall);
+  const int lane = threadIdx.x & 63;
+  for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * kWavesPerBlock) {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+      const int k = tile * (64 * kSmall) + j * 64 + lane;
+      if (k < nn) {
+        const int id = near_ids[k];
+        bool isvf;
+        const int4 q = cand_of(id, nvf, cand_vf, cand_ee, isvf);
+        Hit h;
+        const bool hit = ccd_tests(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt, h);
+        hit_flag[k] = (uint8_t)hit;
+        if (hit) c += isvf ? 1 : (1 << 16);
+      }
+    }
+    c = wave_sum(c);
+    if (lane == 0) tile_sum[tile] = c;
+  }
+}
+
+// pass 7 (after the scan of the tile counts): each hit is recomputed and written at
+// its ordered position among the VF or EE contacts.
 __global__ void __launch_bounds__(kBlock) k_ccd_write(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
-    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
-    const int* __restrict__ plan, const float2* __restrict__ cps,
-    const float* __restrict__ roots, const int* __restrict__ off_vf,
-    const int* __restrict__ d_hvf, const int* __restrict__ off_ee, const int* __restrict__ d_hee,
-    int64_t vf_out_cap, int64_t ee_out_cap, const float4* __restrict__ x,
-    const float4* __restrict__ v, float th, float dt, int4* __restrict__ vf_v,
-    float4* __restrict__ vf_w, float4* __restrict__ vf_tn, int4* __restrict__ ee_v,
-    float4* __restrict__ ee_w, float4* __restrict__ ee_tn, uint32_t* __restrict__ warn) {
-  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
-  const int hvf = *d_hvf, hee = *d_hee;
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total; id += gridDim.x * kBlock) {
-    const bool isvf = id < nvf;
-    const int k = isvf ? id : id - nvf;
-    const int* off = isvf ? off_vf : off_ee;
-    const int n = isvf ? nvf : total - nvf;
-    const int o = off[k];
-    const int next = k + 1 < n ? off[k + 1] : (isvf ? hvf : hee);
-    if (next == o) continue;
-    if (o >= (isvf ? vf_out_cap : ee_out_cap)) {
-      atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
-      continue;
+    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_nnear,
+    const int* __restrict__ near_ids, const int* __restrict__ plan,
+    const float2* __restrict__ cps, const float* __restrict__ roots,
+    const uint8_t* __restrict__ hit_flag, const int* __restrict__ tile_pre, int64_t vf_out_cap,
+    int64_t ee_out_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
+    float dt, int4* __restrict__ vf_v, float4* __restrict__ vf_w, float4* __restrict__ vf_tn,
+    int4* __restrict__ ee_v, float4* __restrict__ ee_w, float4* __restrict__ ee_tn,
+    uint32_t* __restrict__ warn) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nn = *d_nnear;
+  const int ntiles = (nn + (64 * kSmall) - 1) / (64 * kSmall);
+  const int lane = threadIdx.x & 63;
+  bool over = false;
+  for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * kWavesPerBlock) {
+    int id[kSmall], c[kSmall], off[kSmall];
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+      const int k = tile * (64 * kSmall) + j * 64 + lane;
+      const bool hit = k < nn && hit_flag[k];
+      id[j] = hit ? near_ids[k] : -1;
+      c[j] = hit ? (id[j] < nvf ? 1 : (1 << 16)) : 0;
     }
-    const int4 c = isvf ? cand_vf[k] : cand_ee[k];
-    Hit h;
-    ccd_tests(isvf, c, plan[id], cps[id], id, roots, x, v, th, dt, h);
-    const float4 w4 = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
-    const float4 tn = make_float4(h.t, h.n.x, h.n.y, h.n.z);
-    if (isvf) { vf_v[o] = c; vf_w[o] = w4; vf_tn[o] = tn; }
-    else { ee_v[o] = c; ee_w[o] = w4; ee_tn[o] = tn; }
+    tile_offsets<kSmall>(c, off);
+    const int pvf = tile_pre[2 * tile], pee = tile_pre[2 * tile + 1];
+#pragma unroll
+    for (int j = 0; j < kSmall; ++j) {
+      if (id[j] < 0) continue;
+      bool isvf;
+      const int4 q = cand_of(id[j], nvf, cand_vf, cand_ee, isvf);
+      const int64_t o = isvf ? (int64_t)pvf + (off[j] & 0xffff) : (int64_t)pee + (off[j] >> 16);
+      if (o >= (isvf ? vf_out_cap : ee_out_cap)) {
+        over = true;
+        continue;
+      }
+      Hit h;
+      ccd_tests(isvf, q, plan[id[j]], cps[id[j]], id[j], roots, x, v, th, dt, h);
+      const float4 w4 = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+      const float4 tn = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+      if (isvf) { vf_v[o] = q; vf_w[o] = w4; vf_tn[o] = tn; }
+      else { ee_v[o] = q; ee_w[o] = w4; ee_tn[o] = tn; }
+    }
   }
+  if (over) atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
 }
 
 __global__ void __launch_bounds__(kBlock) k_dyn_count(const int4* __restrict__ vfv,
@@ -689,9 +894,6 @@ __global__ void k_copy_i32(const int* __restrict__ s, int* __restrict__ d, int n
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) d[i] = s[i];
 }
 
-__global__ void k_min_cap(int* __restrict__ d_n, int64_t cap) {
-  if (threadIdx.x == 0 && (int64_t)*d_n > cap) *d_n = (int)cap;
-}
 
 }  // namespace
 
@@ -702,23 +904,26 @@ static int grid_for(int64_t cap) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
 }
 
+// workgroups for tiles of 64 * R items over up to `items` items
+static int tile_grid(int64_t items, int R) {
+  const int64_t g = (items + kBlock * R - 1) / (kBlock * R);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
+}
+
 int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
                   const float4* v, float th, float dt, int ee_on, hipStream_t s) {
-  const int gp = grid_for(nb.pair_cap);
-  hipLaunchKernelGGL(k_cand_count, dim3(gp), dim3(kBlock), 0, s, nb.pairs, nb.d_npairs,
-                     nb.pair_cap, tri, x, p, th, ee_on, nb.mask, nb.cnt_vf, nb.cnt_ee);
+  // candidates: masks and tile counts, tile prefixes (and the clamped totals), writes
+  const int ptiles = (int)((nb.pair_cap + 64 * kSmall - 1) / (64 * kSmall));
+  hipLaunchKernelGGL(k_cand_count, dim3(tile_grid(nb.pair_cap, kSmall)), dim3(kBlock), 0, s, nb.pairs,
+                     nb.d_npairs, nb.pair_cap, tri, x, p, th, ee_on, nb.mask, nb.tile_sum);
   CM_LAUNCH_CHECK();
-  int rc = scan_exclusive(nb.cnt_vf, nb.cnt_vf, (int)nb.pair_cap, nb.d_npairs, nb.d_ncand_vf,
-                          nb.scan_temp, s);
+  int rc = scan_tiles(nb.d_npairs, 64 * kSmall, ptiles, nb.tile_sum, nb.tile_pre, 1, nb.vf_cap,
+                      nb.ee_cap, nb.d_ncand_vf, nb.d_ncand_ee, s);
   if (rc) return rc;
-  rc = scan_exclusive(nb.cnt_ee, nb.cnt_ee, (int)nb.pair_cap, nb.d_npairs, nb.d_ncand_ee,
-                      nb.scan_temp, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_cand_write, dim3(gp), dim3(kBlock), 0, s, nb.pairs, nb.d_npairs,
-                     nb.pair_cap, tri, nb.mask, nb.cnt_vf, nb.cnt_ee, nb.cand_vf, nb.cand_ee,
+  hipLaunchKernelGGL(k_cand_write, dim3(tile_grid(nb.pair_cap, kSmall)), dim3(kBlock), 0, s, nb.pairs,
+                     nb.d_npairs, nb.pair_cap, tri, nb.mask, nb.tile_pre, nb.cand_vf, nb.cand_ee,
                      nb.vf_cap, nb.ee_cap, nb.warn);
-  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_vf, nb.vf_cap);
-  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_ncand_ee, nb.ee_cap);
+  // CCD: plan, secant roots, tests with tile hit counts, tile prefixes, writes
   const int64_t ccap = nb.vf_cap + nb.ee_cap;
   const int gc = grid_for(ccap);
   hipLaunchKernelGGL(k_ccd_plan, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
@@ -726,22 +931,28 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
                      nb.d_ntotal);
   hipLaunchKernelGGL(k_ccd_secant, dim3(kSecantGrid), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
                      nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, x, v, nb.croot);
-  hipLaunchKernelGGL(k_ccd_test, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
-                     nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, nb.croot, x, v, th,
-                     dt, nb.flag_vf, nb.flag_ee);
+  hipLaunchKernelGGL(k_ccd_near, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
+                     nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp,
+                     nb.croot, x, v, th, dt, nb.hit_flag, nb.tile_sum);
   CM_LAUNCH_CHECK();
-  rc = scan_exclusive(nb.flag_vf, nb.flag_vf, (int)nb.vf_cap, nb.d_ncand_vf, nb.d_nvf,
-                      nb.scan_temp, s);
+  const int ctiles = (int)((ccap + 64 * kBig - 1) / (64 * kBig));
+  rc = scan_tiles(nb.d_ntotal, 64 * kBig, ctiles, nb.tile_sum, nb.tile_pre, 0, ccap, 0,
+                  nb.d_nnear, nullptr, s);
   if (rc) return rc;
-  rc = scan_exclusive(nb.flag_ee, nb.flag_ee, (int)nb.ee_cap, nb.d_ncand_ee, nb.d_nee,
-                      nb.scan_temp, s);
+  hipLaunchKernelGGL(k_near_list, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.d_ntotal,
+                     nb.hit_flag, nb.tile_pre, nb.near_ids);
+  hipLaunchKernelGGL(k_ccd_test, dim3(tile_grid(ccap, kSmall)), dim3(kBlock), 0, s, nb.cand_vf,
+                     nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,
+                     nb.ccp, nb.croot, x, v, th, dt, nb.hit_flag, nb.tile_sum);
+  CM_LAUNCH_CHECK();
+  const int ntiles_near = (int)((ccap + 64 * kSmall - 1) / (64 * kSmall));
+  rc = scan_tiles(nb.d_nnear, 64 * kSmall, ntiles_near, nb.tile_sum, nb.tile_pre, 1, nb.vf_out_cap,
+                  nb.ee_out_cap, nb.d_nvf, nb.d_nee, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_ccd_write, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
-                     nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, nb.croot,
-                     nb.flag_vf, nb.d_nvf, nb.flag_ee, nb.d_nee, nb.vf_out_cap, nb.ee_out_cap, x,
+  hipLaunchKernelGGL(k_ccd_write, dim3(tile_grid(ccap, kSmall)), dim3(kBlock), 0, s, nb.cand_vf,
+                     nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,
+                     nb.ccp, nb.croot, nb.hit_flag, nb.tile_pre, nb.vf_out_cap, nb.ee_out_cap, x,
                      v, th, dt, nb.vf_v, nb.vf_w, nb.vf_tn, nb.ee_v, nb.ee_w, nb.ee_tn, nb.warn);
-  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_nvf, nb.vf_out_cap);
-  hipLaunchKernelGGL(k_min_cap, dim3(1), dim3(64), 0, s, nb.d_nee, nb.ee_out_cap);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

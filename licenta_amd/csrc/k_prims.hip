@@ -255,6 +255,99 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(
   }
 }
 
+// ---- scan of per-tile sums ----------------------------------------------------------
+// Kernels that produce counts in tiles of consecutive items also write each tile's sum;
+// this one workgroup scans the sums into tile prefixes, and the consuming kernel
+// adds the exclusive scan within its tile. The counts are then read once instead
+// of three times. A packed sum holds two counts below 2^16 (low and high half),
+// scanned separately; the totals are clamped to the buffers they index.
+constexpr int kTileScanBlock = 1024;
+constexpr int kTileScanChunk = 8192;  // sums staged in LDS per round (two 32 KiB halves)
+
+__global__ void __launch_bounds__(kTileScanBlock) k_scan_tiles(
+    const int* __restrict__ d_items, int per_tile, int ntile_max, const int* __restrict__ sums,
+    int* __restrict__ pre, int packed, int64_t cap0, int64_t cap1, int* __restrict__ d_tot0,
+    int* __restrict__ d_tot1) {
+  // Each round stages up to kTileScanChunk sums in LDS with coalesced loads, scans
+  // them (8 consecutive sums per thread, then across the workgroup) and writes the
+  // prefixes back with coalesced stores.
+  __shared__ int sa[kTileScanChunk], sb[kTileScanChunk];
+  __shared__ int lds[2][kTileScanBlock / 64];
+  constexpr int kPer = kTileScanChunk / kTileScanBlock;  // 8
+  const int items = d_items ? max(0, *d_items) : ntile_max * per_tile;
+  const int nt = min(ntile_max, (items + per_tile - 1) / per_tile);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int carry_a = 0, carry_b = 0;
+  for (int c0 = 0; c0 < nt; c0 += kTileScanChunk) {
+    const int len = min(kTileScanChunk, nt - c0);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = k * kTileScanBlock + (int)threadIdx.x;
+      const int v = i < len ? sums[c0 + i] : 0;
+      sa[i] = packed ? (v & 0xffff) : v;
+      sb[i] = packed ? (v >> 16) : 0;
+    }
+    __syncthreads();
+    const int base = (int)threadIdx.x * kPer;
+    int a = 0, b = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      a += sa[base + k];
+      b += sb[base + k];
+    }
+    int ia = a, ib = b;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int ta = __shfl_up(ia, o, 64), tb = __shfl_up(ib, o, 64);
+      if (lane >= o) {
+        ia += ta;
+        ib += tb;
+      }
+    }
+    if (lane == 63) {
+      lds[0][wid] = ia;
+      lds[1][wid] = ib;
+    }
+    __syncthreads();
+    int ra = carry_a + ia - a, rb = carry_b + ib - b;
+    for (int w = 0; w < kTileScanBlock / 64; ++w) {
+      const int ta = lds[0][w], tb = lds[1][w];
+      if (w < wid) {
+        ra += ta;
+        rb += tb;
+      }
+      carry_a += ta;
+      carry_b += tb;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int va = sa[base + k], vb = sb[base + k];
+      sa[base + k] = ra;
+      sb[base + k] = rb;
+      ra += va;
+      rb += vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = k * kTileScanBlock + (int)threadIdx.x;
+      if (i < len) {
+        if (packed) {
+          pre[2 * (c0 + i)] = sa[i];
+          pre[2 * (c0 + i) + 1] = sb[i];
+        } else {
+          pre[c0 + i] = sa[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (d_tot0) *d_tot0 = (int)min<int64_t>(carry_a, cap0);
+    if (packed && d_tot1) *d_tot1 = (int)min<int64_t>(carry_b, cap1);
+  }
+}
+
 }  // namespace
 
 static int scan_tiles(int64_t n_max) {
@@ -276,6 +369,14 @@ int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_to
   const int grid = std::min(scan_tiles(n_max), kScanMaxGrid);
   hipLaunchKernelGGL(k_scan, dim3(grid), dim3(kBlock), 0, s, in, out, n_max, d_n, d_total, ctr,
                      state);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int scan_tiles(const int* d_items, int per_tile, int ntile_max, const int* sums, int* pre,
+               int packed, int64_t cap0, int64_t cap1, int* d_tot0, int* d_tot1, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kTileScanBlock), 0, s, d_items, per_tile,
+                     ntile_max, sums, pre, packed, cap0, cap1, d_tot0, d_tot1);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

@@ -66,7 +66,9 @@ def _compare(batch, singles, scenes):
 def test_batch_matches_single_bodies():
     from licenta_amd.body import ClothBody
     scenes = _scenes()
-    params = dict(scenes[0]["params"], thickness=0.004)
+    # default candidate budgets: the crushed cloth needs more candidates per triangle
+    # than the reduced budgets of the many-cloth benchmark scene allow
+    params = dict(scenes[0]["params"], thickness=0.004, reserved=(0, 0, 0))
     singles = [ClothBody.from_scene(dict(s, params=dict(params, k_stretch=s["params"]["k_stretch"],
                                                         k_bend=s["params"]["k_bend"])))
                for s in scenes]
@@ -88,6 +90,8 @@ def test_batch_matches_single_bodies():
         total += c["vf"] + c["ee"]
     assert total > 0 and batch.counts()["pairs"] > 0
     assert batch.warnings() & 4, "the crushed cloth should build a broken tree"
+    # no capacity was exceeded, so every body kept all of its candidates and contacts
+    assert not batch.warnings() & 16 and not any(b.warnings() & 16 for b in singles)
     batch.close()
     for body in singles:
         body.close()
