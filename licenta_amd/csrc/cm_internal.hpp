@@ -163,7 +163,7 @@ int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s);
 // ---- PBD (k_pbd.hip) -------------------------------------------------------
 // Vertex tiles of the Jacobi kernel: at most kTileV consecutive vertices holding at
 // most kTileSlots constraint slots. Each tile lists the constraints touching it once.
-constexpr int kTileV = 64;
+constexpr int kTileV = 128;
 constexpr int kTileSlots = 3072;
 struct alignas(16) TileEntry {
   int4 ids;            // vertex ids; ids.z < 0 marks a distance constraint (ids.x, ids.y)

@@ -11,8 +11,8 @@
 //  k_solve        <- one Jacobi iteration of PBDStepSolver (:648-659): ground, LRA,
 //                    static (stretch :986-1022, bend :1024-1147), dynamic (VF :1212-1311,
 //                    EE :1313-1415) corrections and FinalCorrectionStep (:1690-1698)
-//                    in one launch: a workgroup evaluates the slots of 64 consecutive
-//                    vertices in parallel into LDS, then each vertex sums its slots in
+//                    in one launch: a workgroup evaluates the slots of a tile of up to
+//                    128 consecutive vertices in parallel into LDS, then each vertex sums its slots in
 //                    the reference's slot order, so the per-vertex sums are the
 //                    reference's gathers (:1417-1475) without the global slot buffer.
 //  k_integrate    <- _FinalUpdate (:1916-1932) + _WorldConstraints
@@ -316,10 +316,9 @@ __device__ f3 bend_corr(int4 ids, float2 kpphi0, int role, int abs_mode,
   }
 }
 
-// stretch_corr for both roles at once (same operations as stretch_corr)
-__device__ __forceinline__ void stretch_both(int2 ids, float2 l0kp, const float4* __restrict__ p,
-                                             f3& c0, f3& c1) {
-  const float4 a = p[ids.x], b = p[ids.y];
+// stretch_corr for both roles at once (same operations as stretch_corr), from the
+// two positions already loaded
+__device__ __forceinline__ void stretch_both(float4 a, float4 b, float2 l0kp, f3& c0, f3& c1) {
   c0 = c1 = mk3(0, 0, 0);
   const float w1 = a.w, w2 = b.w, ws = w1 + w2;
   if (ws == 0) return;
@@ -332,12 +331,12 @@ __device__ __forceinline__ void stretch_both(int2 ids, float2 l0kp, const float4
   c1 = mul(tt, w2);
 }
 
-// bend_corr for all four roles at once (same operations as bend_corr)
+// bend_corr for all four roles at once (same operations as bend_corr), from the four
+// positions already loaded
 template <class T>
-__device__ __forceinline__ void bend_all(int4 ids, float2 kpphi0, int abs_mode,
-                                         const float4* __restrict__ p, f3 c[4]) {
+__device__ __forceinline__ void bend_all(float4 a1, float4 a2, float4 a3, float4 a4,
+                                         float2 kpphi0, int abs_mode, f3 c[4]) {
   c[0] = c[1] = c[2] = c[3] = mk3(0, 0, 0);
-  const float4 a1 = p[ids.x], a2 = p[ids.y], a3 = p[ids.z], a4 = p[ids.w];
   const T w1 = (T)a1.w, w2 = (T)a2.w, w3 = (T)a3.w, w4 = (T)a4.w;
   tv<T> p2 = tfrom<T>(a2), p3 = tfrom<T>(a3), p4 = tfrom<T>(a4);
   if (!abs_mode) {
@@ -382,11 +381,10 @@ __device__ __forceinline__ void bend_all(int4 ids, float2 kpphi0, int abs_mode,
   c[3] = tto(tmul(q4, -(w4 * ts)));
 }
 
-__device__ __forceinline__ f3 vf_corr(int4 v, float4 w, int role, float th,
-                                      const float4* __restrict__ p) {
+__device__ __forceinline__ f3 vf_corr(float4 A, float4 B0, float4 B1, float4 B2, float4 w,
+                                      int role, float th) {
   const float b0 = fclamp(-w.y, 0.f, 1.f), b1 = fclamp(-w.z, 0.f, 1.f),
               b2 = fclamp(-w.w, 0.f, 1.f);
-  const float4 A = p[v.x], B0 = p[v.y], B1 = p[v.z], B2 = p[v.w];
   const float ww = A.w, w0 = B0.w, w1 = B1.w, w2 = B2.w;
   const f3 q = add(add(mul(xyz(B0), b0), mul(xyz(B1), b1)), mul(xyz(B2), b2));
   f3 nn = sub(xyz(A), q);
@@ -405,10 +403,9 @@ __device__ __forceinline__ f3 vf_corr(int4 v, float4 w, int role, float th,
   }
 }
 
-__device__ __forceinline__ f3 ee_corr(int4 v, float4 w, int role, float th,
-                                      const float4* __restrict__ p) {
+__device__ __forceinline__ f3 ee_corr(float4 P0, float4 P1, float4 P2, float4 P3, float4 w,
+                                      int role, float th) {
   const float b0 = w.x, b1 = w.y, b2 = -w.z, b3 = -w.w;
-  const float4 P0 = p[v.x], P1 = p[v.y], P2 = p[v.z], P3 = p[v.w];
   const float w0 = P0.w, w1 = P1.w, w2 = P2.w, w3 = P3.w;
   const f3 q0 = add(mul(xyz(P0), b0), mul(xyz(P1), b1));
   const f3 q1 = add(mul(xyz(P2), b2), mul(xyz(P3), b3));
@@ -462,15 +459,16 @@ __device__ __forceinline__ f3 slot_corr(const SolveArgs& a, uint32_t s,
 }
 
 // one dynamic record's correction (payload: contact ids with role and VF bit, weights)
-__device__ __forceinline__ f3 rec_corr(const SolveArgs& a, const RecPayload& rp,
-                                       const float4* __restrict__ pin) {
+// from the record's four positions already loaded
+__device__ __forceinline__ f3 rec_corr(const RecPayload& rp, const float4 (&P)[4], float th) {
   const int role = (rp.ids.x >> 29) & 3;
-  const int4 ids = make_int4(rp.ids.x & 0x1fffffff, rp.ids.y, rp.ids.z, rp.ids.w);
-  return rp.ids.x < 0 ? vf_corr(ids, rp.w, role, a.th, pin) : ee_corr(ids, rp.w, role, a.th, pin);
+  return rp.ids.x < 0 ? vf_corr(P[0], P[1], P[2], P[3], rp.w, role, th)
+                      : ee_corr(P[0], P[1], P[2], P[3], rp.w, role, th);
 }
 
 constexpr int kSolveV = 64;  // vertices per workgroup (slot-parallel form)
 constexpr int kLds = kTileSlots;  // slot corrections staged in LDS per pass
+constexpr int kBatch = 4;         // constraint entries / records per thread in flight
 
 __device__ __forceinline__ void lds_put(float* cx, float* cy, float* cz, uint32_t off, f3 c) {
   cx[off] = c.x;
@@ -542,23 +540,48 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
       const int c1 = min(c0 + kLds, s1);
       if (kTiles) {  // a tile's slots fit one pass (c0 == s0)
         const int eb = a.eprefix[blockIdx.x], ee = a.eprefix[blockIdx.x + 1];
-        for (int q = eb + tid; q < ee; q += kBlock) {
-          const TileEntry te = a.entries[q];
-          if (te.ids.z < 0) {
-            f3 c0v, c1v;
-            stretch_both(make_int2(te.ids.x, te.ids.y), make_float2(te.a, te.b), pin, c0v, c1v);
-            if ((te.s01 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, te.s01 & 0xffffu, c0v);
-            if ((te.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, te.s01 >> 16, c1v);
-          } else {
-            f3 cr[4];
-            if (a.bend_fp64)
-              bend_all<double>(te.ids, make_float2(te.a, te.b), a.abs_mode, pin, cr);
-            else
-              bend_all<float>(te.ids, make_float2(te.a, te.b), a.abs_mode, pin, cr);
-            if ((te.s01 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, te.s01 & 0xffffu, cr[0]);
-            if ((te.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, te.s01 >> 16, cr[1]);
-            if ((te.s23 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, te.s23 & 0xffffu, cr[2]);
-            if ((te.s23 >> 16) != 0xffffu) lds_put(cx, cy, cz, te.s23 >> 16, cr[3]);
+        // kBatch entries per thread at a time: their loads, then their positions'
+        // loads, are in flight together before any of them is evaluated
+        for (int q0 = eb + tid; q0 < ee; q0 += kBlock * kBatch) {
+          TileEntry te[kBatch];
+          float4 P[kBatch][4];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u)
+            if (q0 + u * kBlock < ee) te[u] = a.entries[q0 + u * kBlock];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) {
+            if (q0 + u * kBlock >= ee) continue;
+            P[u][0] = pin[te[u].ids.x];
+            P[u][1] = pin[te[u].ids.y];
+            if (te[u].ids.z >= 0) {
+              P[u][2] = pin[te[u].ids.z];
+              P[u][3] = pin[te[u].ids.w];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) {
+            if (q0 + u * kBlock >= ee) continue;
+            const TileEntry& t = te[u];
+            if (t.ids.z < 0) {
+              f3 c0v, c1v;
+              stretch_both(P[u][0], P[u][1], make_float2(t.a, t.b), c0v, c1v);
+              if ((t.s01 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, t.s01 & 0xffffu, c0v);
+              if ((t.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, t.s01 >> 16, c1v);
+            } else {
+              f3 cr[4];
+              if (a.bend_fp64)
+                bend_all<double>(P[u][0], P[u][1], P[u][2], P[u][3], make_float2(t.a, t.b),
+                                 a.abs_mode, cr);
+              else
+                bend_all<float>(P[u][0], P[u][1], P[u][2], P[u][3], make_float2(t.a, t.b),
+                                a.abs_mode, cr);
+              if ((t.s01 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, t.s01 & 0xffffu, cr[0]);
+              if ((t.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, t.s01 >> 16, cr[1]);
+              if ((t.s23 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, t.s23 & 0xffffu, cr[2]);
+              if ((t.s23 >> 16) != 0xffffu) lds_put(cx, cy, cz, t.s23 >> 16, cr[3]);
+            }
+            // evaluate the batch's entries one after the other (bounded registers)
+            __builtin_amdgcn_sched_barrier(0);
           }
         }
       } else {
@@ -593,8 +616,25 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
       int c = 0;
       for (int c0 = r0; c0 < r1; c0 += kLds) {
         const int c1 = min(c0 + kLds, r1);
-        for (int q = c0 + tid; q < c1; q += kBlock)
-          lds_put(cx, cy, cz, q - c0, rec_corr(a, a.rec[q], pin));
+        for (int q0 = c0 + tid; q0 < c1; q0 += kBlock * kBatch) {
+          RecPayload rp[kBatch];
+          float4 P[kBatch][4];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u)
+            if (q0 + u * kBlock < c1) rp[u] = a.rec[q0 + u * kBlock];
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u) {
+            if (q0 + u * kBlock >= c1) continue;
+            P[u][0] = pin[rp[u].ids.x & 0x1fffffff];
+            P[u][1] = pin[rp[u].ids.y];
+            P[u][2] = pin[rp[u].ids.z];
+            P[u][3] = pin[rp[u].ids.w];
+          }
+#pragma unroll
+          for (int u = 0; u < kBatch; ++u)
+            if (q0 + u * kBlock < c1)
+              lds_put(cx, cy, cz, q0 + u * kBlock - c0, rec_corr(rp[u], P[u], a.th));
+        }
         __syncthreads();
         const int kend = min(e, c1);
         for (; k + 4 <= kend; k += 4) {
