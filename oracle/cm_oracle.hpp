@@ -863,6 +863,16 @@ struct Sim {
   std::vector<int32_t> pairs;  // 2 * P
   std::vector<cm_contact> vf, ee;
   uint32_t warn = 0;
+  // differentiable rollout (no reference counterpart): with `record` set, step()
+  // keeps each substep's entering state, the positions at the start of every Jacobi
+  // iteration and after the last, and the substep's contact set
+  struct Tape {
+    std::vector<RV> x, v;
+    std::vector<std::vector<RV>> pk;
+    std::vector<cm_contact> vf, ee;
+  };
+  bool record = false;
+  std::vector<Tape> tape;
   bool sphere_on = false;
   F3 sphere_c{0, 0, 0};
   float sphere_r = 0.f;
@@ -1278,7 +1288,9 @@ struct Sim {
     std::vector<RV> acc(n);
     std::vector<int> cnt(n);
     std::vector<RV> rec(4 * C);
+    std::vector<std::vector<RV>>* pk_out = record ? &tape.back().pk : nullptr;
     for (int it = 0; it < prm.solver_iterations; ++it) {
+      if (pk_out) pk_out->push_back(p);
       std::fill(acc.begin(), acc.end(), mk<R>(0, 0, 0));
       std::fill(cnt.begin(), cnt.end(), 0);
       // ground corrections (Physics/CudaPBD.cu:1594-1629)
@@ -1356,6 +1368,7 @@ struct Sim {
       // FinalCorrectionStep (:1690-1698)
       for (int i = 0; i < n; ++i) p[i] = p[i] + divs(acc[i], R(cnt[i]));
     }
+    if (pk_out) pk_out->push_back(p);
   }
 
   // ---- stage 7: integration + world clamp (Physics/CudaPBD.cu:1916-1932,
@@ -1383,6 +1396,11 @@ struct Sim {
 
   // One substep (Physics/CudaDeformableBody.cpp:550-627).
   void step() {
+    if (record) {
+      tape.emplace_back();
+      tape.back().x = x;
+      tape.back().v = v;
+    }
     std::vector<F3> xf(n), pf(n), vf_(n);
     for (int i = 0; i < n; ++i) {
       xf[i] = cvt<float>(x[i]);
@@ -1399,8 +1417,235 @@ struct Sim {
       predict<float>(xf, vf_, pf);  // FP32 shadow for the discrete stages
     }
     collide(xf, pf, vf_);
+    if (record) {
+      tape.back().vf = vf;
+      tape.back().ee = ee;
+    }
     solve();
     integrate();
+  }
+
+  // ---- adjoint of the recorded rollout (no reference counterpart) ----
+  // The differentiable substep: v' = v + dt g, p0 = x + dt v', the Jacobi iterations
+  // over stretch, bend and the substep's (frozen) VF / EE contacts, then x = p, v =
+  // (p - x) / dt for free vertices (pinned: x kept, v = 0). Ground, LRA, sphere,
+  // damping and the world clamp are not part of it. Counts are piecewise constant.
+  // Vector-Jacobian products below are written out by hand in R.
+  void stretch_vjp(int s, const std::vector<RV>& ps, const std::vector<RV>& a,
+                   std::vector<RV>& la, R& gkp) const {
+    const int ia = tp.sid1[s], ib = tp.sid2[s];
+    const R w1 = w[ia], w2 = w[ib], ws = w1 + w2;
+    if (ws == 0) return;
+    const RV d = ps[ia] - ps[ib];
+    const R l = len(d);
+    if (l <= R(1e-8f)) return;
+    const RV n = divs(d, l);
+    const R l0 = R(tp.sl0[s]), kp = R(tp.skp[s]);
+    // c1 = -w1 f kp / ws, c2 = w2 f kp / ws with f = (l - l0) n
+    const RV g = mul(a[ib], w2) - mul(a[ia], w1);  // dL/d(f kp / ws)
+    const RV f = mul(n, l - l0);
+    gkp += dot(g, f) / ws;
+    const RV gf = mul(g, kp / ws);
+    // df/dd = I - (l0 / l)(I - n n^T)
+    const RV gd = gf - mul(gf - mul(n, dot(n, gf)), l0 / l);
+    la[ia] = la[ia] + gd;
+    la[ib] = la[ib] - gd;
+  }
+
+  template <class T>
+  void bend_vjp_impl(int bi, const std::vector<RV>& ps, const std::vector<RV>& a,
+                     std::vector<RV>& la, R& gkp) const {
+    typedef V3<T> TV;
+    const int id[4] = {tp.bid[0][bi], tp.bid[1][bi], tp.bid[2][bi], tp.bid[3][bi]};
+    const T kp = T(tp.bkp[bi]), phi0 = T(tp.bphi0[bi]);
+    const T wv[4] = {T(w[id[0]]), T(w[id[1]]), T(w[id[2]]), T(w[id[3]])};
+    const bool abs_mode = (prm.ref_compat & CM_RC_BEND_ABS) != 0;
+    TV P2 = cvt<T>(ps[id[1]]), P3 = cvt<T>(ps[id[2]]), P4 = cvt<T>(ps[id[3]]);
+    if (!abs_mode) {
+      const TV p1 = cvt<T>(ps[id[0]]);
+      P2 = P2 - p1;
+      P3 = P3 - p1;
+      P4 = P4 - p1;
+    }
+    const TV c23 = cross(P2, P3), c24 = cross(P2, P4);
+    const T l23 = len(c23), l24 = len(c24);
+    const T eps = T(1e-8f);
+    if (l23 <= eps || l24 <= eps) return;
+    const TV n1 = divs(c23, l23), n2 = divs(c24, l24);
+    const T dd = dot(n1, n2);
+    const T d = fclamp(dd, T(-1), T(1));
+    if (d == T(-1) || d == T(1)) return;  // the branches with constant output
+    const T phi = std::acos(d);
+    const TV A = cross(P3, n2) + mul(cross(n1, P3), d), B = cross(P4, n1) + mul(cross(n2, P4), d);
+    const TV C = cross(P2, n2) + mul(cross(n1, P2), d), D = cross(P2, n1) + mul(cross(n2, P2), d);
+    const TV q3 = divs(C, l23), q4 = divs(D, l24);
+    const TV q2 = -divs(A, l23) - divs(B, l24);
+    const TV q1 = -q2 - q3 - q4;
+    const TV q[4] = {q1, q2, q3, q4};
+    const T r = std::sqrt(T(1) - d * d), e = phi - phi0;
+    const T term = r * e * kp;
+    T sf = 0;
+    for (int k = 0; k < 4; ++k) sf += wv[k] * dot(q[k], q[k]);
+    if (sf <= eps) return;
+    const T ts = term / sf;
+    // reverse sweep: c_k = -w_k ts q_k
+    TV qb[4];
+    T tsb = 0;
+    for (int k = 0; k < 4; ++k) {
+      const TV ak = cvt<T>(a[id[k]]);
+      qb[k] = mul(ak, -wv[k] * ts);
+      tsb -= wv[k] * dot(q[k], ak);
+    }
+    const T termb = tsb / sf, sfb = -tsb * term / (sf * sf);
+    for (int k = 0; k < 4; ++k) qb[k] = qb[k] + mul(q[k], T(2) * wv[k] * sfb);
+    gkp += R(termb * r * e);
+    const T rb = termb * e * kp, eb = termb * r * kp;
+    T db = -eb / r - rb * d / r;  // phi = acos(d), r = sqrt(1 - d^2)
+    // q1 = -q2 - q3 - q4
+    qb[1] = qb[1] - qb[0];
+    qb[2] = qb[2] - qb[0];
+    qb[3] = qb[3] - qb[0];
+    T l23b = 0, l24b = 0;
+    // q2 = -A / l23 - B / l24, q3 = C / l23, q4 = D / l24
+    const TV Ab = mul(qb[1], -T(1) / l23), Bb = mul(qb[1], -T(1) / l24);
+    l23b += dot(qb[1], A) / (l23 * l23);
+    l24b += dot(qb[1], B) / (l24 * l24);
+    const TV Cb = divs(qb[2], l23), Db = divs(qb[3], l24);
+    l23b -= dot(qb[2], C) / (l23 * l23);
+    l24b -= dot(qb[3], D) / (l24 * l24);
+    TV P2b = mk<T>(0, 0, 0), P3b = mk<T>(0, 0, 0), P4b = mk<T>(0, 0, 0);
+    TV n1b = mk<T>(0, 0, 0), n2b = mk<T>(0, 0, 0);
+    // X = U x N + (M x U) d, for the four (U, N, M) = (P3,n2,n1), (P4,n1,n2), (P2,n2,n1), (P2,n1,n2)
+    auto back = [&](const TV& Xb, const TV& U, const TV& N, const TV& M, TV& Ub, TV& Nb, TV& Mb) {
+      Ub = Ub + cross(N, Xb);
+      Nb = Nb + cross(Xb, U);
+      const TV t = mul(Xb, d);
+      Mb = Mb + cross(U, t);
+      Ub = Ub + cross(t, M);
+      db += dot(Xb, cross(M, U));
+    };
+    back(Ab, P3, n2, n1, P3b, n2b, n1b);
+    back(Bb, P4, n1, n2, P4b, n1b, n2b);
+    back(Cb, P2, n2, n1, P2b, n2b, n1b);
+    back(Db, P2, n1, n2, P2b, n1b, n2b);
+    // d = clamp(n1 . n2) (interior here)
+    n1b = n1b + mul(n2, db);
+    n2b = n2b + mul(n1, db);
+    // n = c / l, l = |c|
+    l23b -= dot(n1b, c23) / (l23 * l23);
+    l24b -= dot(n2b, c24) / (l24 * l24);
+    const TV c23b = divs(n1b, l23) + mul(n1, l23b), c24b = divs(n2b, l24) + mul(n2, l24b);
+    // c23 = P2 x P3, c24 = P2 x P4
+    P2b = P2b + cross(P3, c23b) + cross(P4, c24b);
+    P3b = P3b + cross(c23b, P2);
+    P4b = P4b + cross(c24b, P2);
+    la[id[1]] = la[id[1]] + cvt<R>(P2b);
+    la[id[2]] = la[id[2]] + cvt<R>(P3b);
+    la[id[3]] = la[id[3]] + cvt<R>(P4b);
+    if (!abs_mode) la[id[0]] = la[id[0]] - cvt<R>(P2b + P3b + P4b);
+  }
+  void bend_vjp(int bi, const std::vector<RV>& ps, const std::vector<RV>& a, std::vector<RV>& la,
+                R& gkp) const {
+    if ((prm.flags & CM_F_BEND_FP64) && sizeof(R) < sizeof(double))
+      bend_vjp_impl<double>(bi, ps, a, la, gkp);
+    else
+      bend_vjp_impl<R>(bi, ps, a, la, gkp);
+  }
+
+  // VF and EE contacts: c_r = k_r g(nn), g(nn) = nn - th nn / |nn| while |nn| < th,
+  // with fixed weights; nn = sum_r m_r p_r. Roles whose record is zero were neither
+  // summed nor counted, so their cotangents are left out.
+  void contact_vjp(const cm_contact& k, bool isvf, const std::vector<RV>& ps,
+                   const std::vector<RV>& a, const bool used[4], std::vector<RV>& la) const {
+    R m[4], kr[4];
+    const int* v = k.v;
+    if (isvf) {
+      const R b0 = fclamp(R(-k.w[1]), R(0), R(1)), b1 = fclamp(R(-k.w[2]), R(0), R(1)),
+              b2 = fclamp(R(-k.w[3]), R(0), R(1));
+      m[0] = 1; m[1] = -b0; m[2] = -b1; m[3] = -b2;
+      const R S = w[v[0]] + w[v[1]] * b0 * b0 + w[v[2]] * b1 * b1 + w[v[3]] * b2 * b2;
+      if (S == 0) return;
+      kr[0] = -w[v[0]] / S; kr[1] = w[v[1]] * b0 / S; kr[2] = w[v[2]] * b1 / S;
+      kr[3] = w[v[3]] * b2 / S;
+    } else {
+      const R b0 = R(k.w[0]), b1 = R(k.w[1]), b2 = R(-k.w[2]), b3 = R(-k.w[3]);
+      m[0] = b0; m[1] = b1; m[2] = -b2; m[3] = -b3;
+      const R S = w[v[0]] * b0 * b0 + w[v[1]] * b1 * b1 + w[v[2]] * b2 * b2 + w[v[3]] * b3 * b3;
+      if (S == 0) return;
+      kr[0] = -w[v[0]] * b0 / S; kr[1] = -w[v[1]] * b1 / S; kr[2] = w[v[2]] * b2 / S;
+      kr[3] = w[v[3]] * b3 / S;
+    }
+    RV nn = mk<R>(0, 0, 0);
+    for (int r = 0; r < 4; ++r) nn = nn + mul(ps[v[r]], m[r]);
+    const R l = len(nn), th = R(prm.thickness);
+    if (!(l - th < 0)) return;
+    const RV n = divs(nn, l);
+    RV G = mk<R>(0, 0, 0);
+    for (int r = 0; r < 4; ++r)
+      if (used[r]) G = G + mul(a[v[r]], kr[r]);
+    // dg/dnn = I - (th / l)(I - n n^T)
+    const RV gn = G - mul(G - mul(n, dot(n, G)), th / l);
+    for (int r = 0; r < 4; ++r) la[v[r]] = la[v[r]] + mul(gn, m[r]);
+  }
+
+  // Backward sweep over the tape: lx / lv = dL/dx, dL/dv after the last substep in,
+  // dL/dx, dL/dv before the first out; dL/dkp per distance and bend constraint added
+  // to gs / gb.
+  void adjoint(std::vector<RV>& lx, std::vector<RV>& lv, std::vector<R>& gs,
+               std::vector<R>& gb) const {
+    const R dt = R(prm.timestep);
+    const int S = (int)tp.sid1.size(), Bn = (int)tp.bkp.size();
+    gs.assign(S, R(0));
+    gb.assign(Bn, R(0));
+    std::vector<RV> lp(n), a(n), la(n);
+    for (int s = (int)tape.size() - 1; s >= 0; --s) {
+      const Tape& tpe = tape[s];
+      const int I = (int)tpe.pk.size() - 1;
+      for (int i = 0; i < n; ++i) {
+        if (pinned[i]) {
+          lp[i] = mk<R>(0, 0, 0);  // x kept, v = 0
+        } else {
+          lp[i] = lx[i] + divs(lv[i], dt);
+          lx[i] = -divs(lv[i], dt);
+        }
+      }
+      const size_t C = tpe.vf.size() + tpe.ee.size();
+      std::vector<uint8_t> used(4 * C);
+      for (int it = I - 1; it >= 0; --it) {
+        const std::vector<RV>& ps = tpe.pk[it];
+        std::vector<int> cnt(n, 0);
+        for (int i = 0; i < n; ++i) cnt[i] = tp.prefix[i + 1] - tp.prefix[i];
+        for (size_t c = 0; c < C; ++c) {
+          RV cc[4];
+          const bool isvf = c < tpe.vf.size();
+          const cm_contact& k = isvf ? tpe.vf[c] : tpe.ee[c - tpe.vf.size()];
+          if (isvf) contact_vf(k, ps, cc); else contact_ee(k, ps, cc);
+          for (int r = 0; r < 4; ++r) {
+            used[4 * c + r] = !is_zero(cc[r]);
+            cnt[k.v[r]] += used[4 * c + r];
+          }
+        }
+        for (int i = 0; i < n; ++i) {
+          a[i] = divs(lp[i], R(cnt[i]));
+          la[i] = lp[i];
+        }
+        for (int j = 0; j < S; ++j) stretch_vjp(j, ps, a, la, gs[j]);
+        for (int j = 0; j < Bn; ++j) bend_vjp(j, ps, a, la, gb[j]);
+        for (size_t c = 0; c < C; ++c) {
+          const bool isvf = c < tpe.vf.size();
+          const cm_contact& k = isvf ? tpe.vf[c] : tpe.ee[c - tpe.vf.size()];
+          bool u[4];
+          for (int r = 0; r < 4; ++r) u[r] = used[4 * c + r] != 0;
+          contact_vjp(k, isvf, ps, a, u, la);
+        }
+        lp.swap(la);
+      }
+      // p0 = x + dt v', v' = v + dt g
+      for (int i = 0; i < n; ++i) {
+        lx[i] = lx[i] + lp[i];
+        lv[i] = mul(lp[i], dt);
+      }
+    }
   }
 };
 

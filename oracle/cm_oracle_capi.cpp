@@ -295,4 +295,54 @@ int orc_test_ee(const float* x12, float th, cm_contact* c) {
   for (int k = 0; k < 4; ++k) x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
   return test_ee(x[0], x[1], x[2], x[3], *c, th, 0.000000001f);
 }
+
+// ---- differentiable rollout (no reference counterpart) ----
+// Recording on: later substeps keep what the adjoint needs (the tape is cleared).
+int orc_record(void* hp, int on) {
+  Handle* h = (Handle*)hp;
+  with<float>(h, [&](auto& sim) {
+    sim.record = on != 0;
+    sim.tape.clear();
+    return 0;
+  });
+  return 0;
 }
+
+// Adjoint of the recorded substeps. lx, lv: 3N doubles, dL/dx and dL/dv after the
+// last substep in, dL/dx and dL/dv before the first out; gk_dist (S) and gk_bend (B):
+// dL/dkp per distance / bend constraint, times dkp/dk = (1 - k)^(1/I - 1) / I when
+// k_dist / k_bend (the stiffnesses) are given, else dL/dkp.
+int orc_adjoint(void* hp, double* lx, double* lv, double* gk_dist, double* gk_bend,
+                const float* k_dist, const float* k_bend) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) {
+    typedef typename std::remove_reference<decltype(sim)>::type S;
+    typedef typename S::RV RV;
+    typedef decltype(RV().x) R;
+    std::vector<RV> x(sim.n), v(sim.n);
+    for (int i = 0; i < sim.n; ++i) {
+      x[i] = mk<R>(R(lx[3 * i]), R(lx[3 * i + 1]), R(lx[3 * i + 2]));
+      v[i] = mk<R>(R(lv[3 * i]), R(lv[3 * i + 1]), R(lv[3 * i + 2]));
+    }
+    std::vector<R> gs, gb;
+    sim.adjoint(x, v, gs, gb);
+    for (int i = 0; i < sim.n; ++i) {
+      lx[3 * i] = x[i].x; lx[3 * i + 1] = x[i].y; lx[3 * i + 2] = x[i].z;
+      lv[3 * i] = v[i].x; lv[3 * i + 1] = v[i].y; lv[3 * i + 2] = v[i].z;
+    }
+    const double I = sim.prm.solver_iterations;
+    for (size_t j = 0; j < gs.size(); ++j) {
+      double f = 1.0;
+      if (k_dist) f = std::pow(1.0 - (double)k_dist[j], 1.0 / I - 1.0) / I;
+      if (gk_dist) gk_dist[j] = (double)gs[j] * f;
+    }
+    for (size_t j = 0; j < gb.size(); ++j) {
+      double f = 1.0;
+      if (k_bend) f = std::pow(1.0 - (double)k_bend[j], 1.0 / I - 1.0) / I;
+      if (gk_bend) gk_bend[j] = (double)gb[j] * f;
+    }
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -39,6 +39,8 @@ def lib():
     L.orc_time_steps.argtypes = [vp, i32]
     L.orc_time_steps.restype = C.c_double
     L.orc_set_sphere.argtypes = [vp, vp, C.c_float]
+    L.orc_record.argtypes = [vp, i32]
+    L.orc_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.orc_set_state.argtypes = [vp, vp, vp, vp, vp]
     L.orc_get_state.argtypes = [vp, vp, vp, vp]
     L.orc_set_parents.argtypes = [vp, vp]
@@ -120,6 +122,26 @@ class OracleSim:
         p = None if p is None else np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 3)
         tri = None if tri is None else np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 4)
         lib().orc_set_state(self.h, _p(x), _p(p), _p(v), _p(tri))
+
+    def record(self, on: bool = True) -> None:
+        """Keep what the adjoint needs from the following substeps (clears the tape)."""
+        lib().orc_record(self.h, int(on))
+
+    def adjoint(self, dl_dx, dl_dv=None, k_dist=None, k_bend=None):
+        """Backward sweep over the recorded substeps. dl_dx / dl_dv: (N, 3) cotangents of
+        the final positions / velocities. Returns (dL/dx0, dL/dv0, dL/dk_dist,
+        dL/dk_bend); the stiffness gradients are per constraint, w.r.t. k when k_dist /
+        k_bend (the stiffnesses used) are given, else w.r.t. kp = 1 - (1 - k)^(1/I)."""
+        c = self.counts()
+        lx = np.ascontiguousarray(dl_dx, dtype=np.float64).reshape(-1, 3).copy()
+        lv = (np.zeros_like(lx) if dl_dv is None
+              else np.ascontiguousarray(dl_dv, dtype=np.float64).reshape(-1, 3).copy())
+        gs = np.zeros(c["s"], np.float64)
+        gb = np.zeros(c["b"], np.float64)
+        kd = None if k_dist is None else np.ascontiguousarray(k_dist, dtype=np.float32)
+        kb = None if k_bend is None else np.ascontiguousarray(k_bend, dtype=np.float32)
+        lib().orc_adjoint(self.h, _p(lx), _p(lv), _p(gs), _p(gb), _p(kd), _p(kb))
+        return lx, lv, gs, gb
 
     def set_parents(self, parents):
         parents = np.ascontiguousarray(parents, dtype=np.int32)
