@@ -227,6 +227,34 @@ int cm_time_kernel(cm_body* body, int32_t which, int32_t reps, float* avg_ms);
 #define CM_NUM_STAGES 8
 int cm_stage_times(cm_body* body, int32_t substeps, float* ms);
 
+/* ---- differentiable rollout ------------------------------------------------
+ * No counterpart in the reference (its substep is not differentiated); for callers
+ * that fit per-constraint stiffness and initial velocities by gradient descent
+ * (e.g. a torch.autograd.Function over this library). The differentiable substep
+ * is prediction, the Jacobi iterations over the distance and bend constraints and
+ * the substep's VF / EE contacts (frozen: contact sets and Jacobi counts do not
+ * carry gradient), and the velocity update; the Morton / sort / LBVH / CCD stages
+ * carry no gradient. Bodies with ground, long-range attachments, a sphere,
+ * damping or friction are rejected (CM_ERR_ARG). */
+
+/* Replace the stiffness of every distance (k_dist, S floats) and bend (k_bend,
+ * B floats) constraint, in [0, 1]; either may be NULL (kept). Per-iteration
+ * stiffness is kp = 1 - (1 - k)^(1/iterations), as at create time
+ * (Physics/CudaPBD.cu:990, 1029). Syncs; invalidates a recorded forward. */
+int cm_set_stiffness(cm_body* body, const float* k_dist, const float* k_bend);
+/* `substeps` substeps from the current state, as cm_step, keeping the state every
+ * `every` substeps (and after the last) for cm_adjoint. Does not sync. */
+int cm_forward_ckpt(cm_body* body, int32_t substeps, int32_t every);
+/* Gradients of a loss L over the last cm_forward_ckpt: in, dl_dx / dl_dv = dL/dx,
+ * dL/dv of the final positions / velocities (3N floats; dl_dv may be NULL = 0);
+ * out (each may be NULL), dL/dx and dL/dv of the state before the first substep
+ * (3N), and dL/dk per distance (S) and bend (B) constraint through kp(k) (infinite
+ * at k = 1). Each segment of `every` substeps is recomputed from its checkpoint,
+ * keeping its Jacobi iterates and contact records, then swept backwards. Leaves
+ * the body in its state after the forward. Syncs. CM_ERR_STATE without a forward. */
+int cm_adjoint(cm_body* body, const float* dl_dx, const float* dl_dv, float* dl_dx0,
+               float* dl_dv0, float* dl_dk_dist, float* dl_dk_bend);
+
 /* ---- unit-level LBVH entry points on caller device buffers -------------- */
 /* Swept leaf AABBs (written to leaf slots [T-1, 2T-1) of d_aabb_min/max) and
  * Morton keys for triangles d_tri in their current order. */

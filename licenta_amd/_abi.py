@@ -119,6 +119,9 @@ def lib():
     L.cm_lbvh_pairs.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, C.c_size_t, vp]
     L.cm_time_kernel.argtypes = [vp, i32, i32, C.POINTER(C.c_float)]
     L.cm_stage_times.argtypes = [vp, i32, vp]
+    L.cm_set_stiffness.argtypes = [vp, vp, vp]
+    L.cm_forward_ckpt.argtypes = [vp, i32, i32]
+    L.cm_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -137,5 +140,6 @@ EXPORTS = (
     "cm_set_state", "cm_get_counts", "cm_get_buffer", "cm_get_warnings", "cm_query_pairs",
     "cm_query_contacts", "cm_tri_update", "cm_lbvh_sort_temp_bytes", "cm_lbvh_sort",
     "cm_lbvh_build_temp_bytes", "cm_lbvh_build", "cm_lbvh_refit", "cm_lbvh_pairs_temp_bytes",
-    "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times",
+    "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times", "cm_set_stiffness", "cm_forward_ckpt",
+    "cm_adjoint",
 )

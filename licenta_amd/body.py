@@ -179,5 +179,34 @@ class ClothBody:
     def pairs(self) -> np.ndarray:
         return self.buffer("pairs")
 
+    # ---- differentiable rollout ------------------------------------------------------
+    def set_stiffness(self, k_dist=None, k_bend=None) -> None:
+        """Per-constraint distance / bend stiffness in [0, 1] (None: kept)."""
+        kd = None if k_dist is None else np.ascontiguousarray(k_dist, dtype=np.float32)
+        kb = None if k_bend is None else np.ascontiguousarray(k_bend, dtype=np.float32)
+        c = self.counts()
+        if kd is not None and kd.shape != (c["s"],):
+            raise ValueError(f"k_dist needs {c['s']} values")
+        if kb is not None and kb.shape != (c["b"],):
+            raise ValueError(f"k_bend needs {c['b']} values")
+        check(lib().cm_set_stiffness(self.h, ptr(kd), ptr(kb)))
+
+    def forward_ckpt(self, substeps: int, every: int) -> None:
+        """`substeps` substeps keeping a checkpoint every `every` for adjoint()."""
+        check(lib().cm_forward_ckpt(self.h, int(substeps), int(every)))
+
+    def adjoint(self, dl_dx, dl_dv=None):
+        """Gradients of a loss over the last forward_ckpt, given dL/dx (and dL/dv) of
+        the final state: (dL/dx0, dL/dv0, dL/dk_dist, dL/dk_bend)."""
+        c = self.counts()
+        lx = np.ascontiguousarray(dl_dx, dtype=np.float32).reshape(self.n, 3)
+        lv = None if dl_dv is None else np.ascontiguousarray(dl_dv, dtype=np.float32).reshape(self.n, 3)
+        gx = np.zeros((self.n, 3), dtype=np.float32)
+        gv = np.zeros((self.n, 3), dtype=np.float32)
+        gkd = np.zeros(c["s"], dtype=np.float32)
+        gkb = np.zeros(c["b"], dtype=np.float32)
+        check(lib().cm_adjoint(self.h, ptr(lx), ptr(lv), ptr(gx), ptr(gv), ptr(gkd), ptr(gkb)))
+        return gx, gv, gkd, gkb
+
     def contacts(self):
         return self.buffer("vf"), self.buffer("ee")

@@ -31,6 +31,45 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 using namespace cm;
 
+namespace {
+
+// Host-built topology; kept by the body so that stiffness changes can rebuild the
+// tile entries.
+struct HostTopo {
+  std::vector<int2> sid;
+  std::vector<float2> s_l0kp;
+  std::vector<int4> bid;
+  std::vector<float2> b_kpphi0;
+  std::vector<int> prefix;
+  std::vector<uint32_t> vslot;
+  std::vector<int4> tri;
+  std::vector<int> lra_fixed;
+  std::vector<float> lra_l0;
+  std::vector<int> tstart, eprefix;  // Jacobi tiles (empty: slot-parallel solve)
+  std::vector<TileEntry> entries;
+  std::vector<int> ecid;  // per entry: constraint index, bit 31 set for a bend
+  std::vector<float> k_dist, k_bend;  // stiffness per constraint (before the kp mapping)
+};
+
+// Differentiable rollout: checkpoints of the state every `every` substeps, and the
+// tape of one segment (each substep's Jacobi iterates and contact records).
+struct AdjState {
+  std::vector<void*> allocs;
+  int nsub = 0, every = 0, nck = 0, iters = 0;
+  bool ready = false;  // a forward run is on record
+  float4 *ck_x = nullptr, *ck_v = nullptr, *ck_p = nullptr;  // (nck + 1) * n; last = final
+  int4* ck_tri = nullptr;                                      // (nck + 1) * t
+  float4* pk = nullptr;       // every * (iters + 1) * n
+  int* dpre = nullptr;        // every * (n + 1)
+  RecPayload* arena = nullptr;
+  int64_t arena_cap = 0;
+  int* ctl = nullptr;  // 2 + every
+  AdjBufs ab;
+  int* ecid = nullptr;
+};
+
+}  // namespace
+
 struct cm_body {
   cm_params prm;
   int n = 0, t = 0, S = 0, B = 0, F = 0, nslots = 0, cap = 0;
@@ -81,6 +120,8 @@ struct cm_body {
   std::vector<uint8_t> pinned_h;
   // counts of the host-built topology (for tests)
   std::vector<int> h_sid, h_bid, h_prefix;
+  HostTopo topo;
+  AdjState adj;
 };
 
 namespace {
@@ -157,20 +198,6 @@ inline uint64_t edge_key(int a, int b) {
   return ((uint64_t)lo << 32) | hi;
 }
 
-struct HostTopo {
-  std::vector<int2> sid;
-  std::vector<float2> s_l0kp;
-  std::vector<int4> bid;
-  std::vector<float2> b_kpphi0;
-  std::vector<int> prefix;
-  std::vector<uint32_t> vslot;
-  std::vector<int4> tri;
-  std::vector<int> lra_fixed;
-  std::vector<float> lra_l0;
-  std::vector<int> tstart, eprefix;  // Jacobi tiles (empty: slot-parallel solve)
-  std::vector<TileEntry> entries;
-};
-
 // Cut the vertices into tiles of at most kTileV consecutive vertices and kTileSlots
 // slots, and list for each tile every constraint that touches it once, with the
 // tile-local slot of each of its roles. Leaves the tiles empty when one vertex has
@@ -188,6 +215,7 @@ void build_tiles(int n, HostTopo& h) {
   std::vector<int> seen_s(h.sid.size(), -1), seen_b(h.bid.size(), -1);  // entry of this tile
   std::vector<int> ep{0};
   std::vector<TileEntry> ent;
+  std::vector<int> cid;
   for (int tile = 0; tile < nt; ++tile) {
     const int s0 = pre[ts[tile]], s1 = pre[ts[tile + 1]];
     const int base = (int)ent.size();
@@ -210,6 +238,7 @@ void build_tiles(int n, HostTopo& h) {
         }
         e.s01 = e.s23 = 0xffffffffu;
         ent.push_back(e);
+        cid.push_back(bend ? (int)(0x80000000u | (uint32_t)idx) : idx);
       }
       TileEntry& e = ent[seen];
       const uint32_t off = (uint32_t)(k - s0);
@@ -218,13 +247,24 @@ void build_tiles(int n, HostTopo& h) {
       word = (word & ~(0xffffu << sh)) | (off << sh);
     }
     // distance constraints first, then bends: a wave then mostly runs one path
-    std::stable_partition(ent.begin() + base, ent.end(),
-                          [](const TileEntry& e) { return e.ids.z < 0; });
+    std::vector<int> order((size_t)ent.size() - base);
+    for (size_t k = 0; k < order.size(); ++k) order[k] = base + (int)k;
+    std::stable_partition(order.begin(), order.end(),
+                          [&](int k) { return ent[k].ids.z < 0; });
+    std::vector<TileEntry> te(order.size());
+    std::vector<int> tc(order.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+      te[k] = ent[order[k]];
+      tc[k] = cid[order[k]];
+    }
+    std::copy(te.begin(), te.end(), ent.begin() + base);
+    std::copy(tc.begin(), tc.end(), cid.begin() + base);
     ep.push_back((int)ent.size());
   }
   h.tstart = std::move(ts);
   h.eprefix = std::move(ep);
   h.entries = std::move(ent);
+  h.ecid = std::move(cid);
 }
 
 int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* tri, int t,
@@ -248,6 +288,7 @@ int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* 
         const int bi = (int)h.bid.size();
         h.bid.push_back(make_int4(q[0], q[1], q[2], q[3]));
         const float kb = k_bend ? k_bend[bi] : P.k_bend;
+        h.k_bend.push_back(kb);
         const float phi0 = (P.ref_compat & CM_RC_BEND_ABS) ? 3.14159265359f
                                                            : dihedral_rest(xyz, q[0], q[1], q[2], q[3]);
         h.b_kpphi0.push_back(make_float2(kp_of(kb, I), phi0));
@@ -276,9 +317,11 @@ int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* 
   }
   h.sid = s_edges;
   h.s_l0kp.resize(S);
-  for (int i = 0; i < S; ++i)
-    h.s_l0kp[i] = make_float2(host_dist(xyz, s_edges[i].x, s_edges[i].y),
-                              kp_of(k_dist ? k_dist[i] : P.k_stretch, I));
+  h.k_dist.resize(S);
+  for (int i = 0; i < S; ++i) {
+    h.k_dist[i] = k_dist ? k_dist[i] : P.k_stretch;
+    h.s_l0kp[i] = make_float2(host_dist(xyz, s_edges[i].x, s_edges[i].y), kp_of(h.k_dist[i], I));
+  }
   // slots: every stretch end in constraint order, then every bend vertex
   // (Physics/CudaPBD.cu:146-233)
   std::vector<int> cnt(n, 0);
@@ -385,9 +428,17 @@ PbdParams pbd_params(const cm_body* b) {
   return pp;
 }
 
+// Where a recorded substep keeps its Jacobi iterates and contact records.
+struct TapeSlot {
+  float4* pk;
+  int* dpre;
+  int slot;
+};
+
 // One substep on b->stream (Physics/CudaDeformableBody.cpp:550-627 stage order).
 // ev (optional): CM_NUM_STAGES + 1 events recorded at the stage boundaries.
-int record_substep(cm_body* b, hipEvent_t* ev = nullptr) {
+// tape (optional): the same substep, keeping what its adjoint needs.
+int record_substep(cm_body* b, hipEvent_t* ev = nullptr, const TapeSlot* tape = nullptr) {
   hipStream_t s = b->stream;
   const cm_params& P = b->prm;
   PbdBufs pb = pbd_bufs(b);
@@ -441,14 +492,22 @@ int record_substep(cm_body* b, hipEvent_t* ev = nullptr) {
     if (rc) return rc;
     rc = launch_copy_i4(b->t, b->tri_alt, b->tri, s);
     if (rc) return rc;
+    if (tape) {
+      CM_HIP(hipMemcpyAsync(tape->dpre, b->nb.dprefix, sizeof(int) * ((size_t)b->n + 1),
+                            hipMemcpyDeviceToDevice, s));
+      rc = launch_tape_records(b->n, b->nb.dprefix, b->nb.rec, b->adj.arena, b->adj.arena_cap,
+                               b->adj.ctl, tape->slot, s);
+      if (rc) return rc;
+    }
   }
   NarrowBufs nb = b->nb;
   if (!coll) nb.dprefix = nullptr;
   mark(CM_STAGE_SOLVER);
-  rc = launch_solver(pb, pp, nb, s);
+  rc = tape ? launch_solver_rec(pb, pp, nb, tape->pk, s) : launch_solver(pb, pp, nb, s);
   if (rc) return rc;
   mark(CM_STAGE_INTEGRATE);
-  rc = launch_integrate(pb, pp, s);
+  rc = tape ? launch_integrate_from(pb, pp, tape->pk + (int64_t)pp.iters * b->n, s)
+            : launch_integrate(pb, pp, s);
   mark(CM_NUM_STAGES);
   return rc;
 }
@@ -589,11 +648,15 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
       for (int i = batch->vstart[c]; i < batch->vstart[c + 1]; ++i) cloth_of[i] = c;
     const int I = b->prm.solver_iterations;
     if (batch->k_stretch)
-      for (size_t i = 0; i < h.sid.size(); ++i)
-        h.s_l0kp[i].y = kp_of(batch->k_stretch[cloth_of[h.sid[i].x]], I);
+      for (size_t i = 0; i < h.sid.size(); ++i) {
+        h.k_dist[i] = batch->k_stretch[cloth_of[h.sid[i].x]];
+        h.s_l0kp[i].y = kp_of(h.k_dist[i], I);
+      }
     if (batch->k_bend)
-      for (size_t i = 0; i < h.bid.size(); ++i)
-        h.b_kpphi0[i].x = kp_of(batch->k_bend[cloth_of[h.bid[i].x]], I);
+      for (size_t i = 0; i < h.bid.size(); ++i) {
+        h.k_bend[i] = batch->k_bend[cloth_of[h.bid[i].x]];
+        h.b_kpphi0[i].x = kp_of(h.k_bend[i], I);
+      }
     build_tiles(n, h);  // tile entries carry the stiffness
   }
   b->S = (int)h.sid.size();
@@ -628,7 +691,8 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
       (rc = upload(b, &b->vslot, h.vslot)) || (rc = upload(b, &b->lra_fixed, h.lra_fixed)) ||
       (!h.tstart.empty() && ((rc = upload(b, &b->tstart, h.tstart)) ||
                              (rc = upload(b, &b->eprefix, h.eprefix)) ||
-                             (rc = upload(b, &b->entries, h.entries)))) ||
+                             (rc = upload(b, &b->entries, h.entries)) ||
+                             (rc = upload(b, &b->adj.ecid, h.ecid)))) ||
       (rc = upload(b, &b->lra_l0, h.lra_l0)) ||
       (rc = dalloc(b, &b->lra_flag, (size_t)n * std::max(b->F, 1))) ||
       (rc = dalloc(b, &b->ground, n)) ||
@@ -743,6 +807,7 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
   if (params->flags & CM_F_SPHERE) {
     b->prm.flags |= CM_F_SPHERE;
   }
+  b->topo = std::move(h);
   CM_HIP(hipDeviceSynchronize());
   *out = b;
   return CM_OK;
@@ -833,6 +898,7 @@ int cm_destroy(cm_body* b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   for (void* p : b->allocs) hipFree(p);
+  for (void* p : b->adj.allocs) hipFree(p);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return CM_OK;
@@ -1139,6 +1205,266 @@ int cm_query_contacts(cm_body* b, cm_contact* vf, int32_t cap_vf, int32_t* n_vf,
   if (n_vf) *n_vf = (int32_t)a;
   if (n_ee) *n_ee = (int32_t)c;
   return rc;
+}
+
+}  // extern "C"
+
+// ---- differentiable rollout (no reference counterpart) ----------------------------
+namespace {
+
+template <class T>
+int adj_alloc(cm_body* b, T** out, size_t count) {
+  *out = nullptr;
+  if (count == 0) count = 1;
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, sizeof(T) * count);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc(") + std::to_string(sizeof(T) * count) +
+              " bytes): " + hipGetErrorString(e));
+    return CM_ERR_ALLOC;
+  }
+  b->adj.allocs.push_back(ptr);
+  *out = (T*)ptr;
+  return CM_OK;
+}
+
+void adj_release(cm_body* b) {
+  AdjState& a = b->adj;
+  for (void* ptr : a.allocs) hipFree(ptr);
+  int* ecid = a.ecid;
+  a = AdjState();
+  a.ecid = ecid;
+  a.ab.ecid = ecid;
+}
+
+int adj_check(cm_body* b, const char* who) {
+  const uint32_t off = CM_F_GROUND | CM_F_LRA | CM_F_SPHERE | CM_F_DAMPING | CM_F_FRICTION;
+  if (b->prm.flags & off) {
+    set_error(std::string(who) + ": the differentiable substep has no ground, long-range "
+                                 "attachments, sphere, damping or friction");
+    return CM_ERR_ARG;
+  }
+  if (b->ntiles <= 0) {
+    set_error(std::string(who) + ": a vertex has more constraint slots than a Jacobi tile holds");
+    return CM_ERR_ARG;
+  }
+  return CM_OK;
+}
+
+// buffers for a forward of nsub substeps with a checkpoint every `every`
+int adj_setup(cm_body* b, int nsub, int every) {
+  AdjState& a = b->adj;
+  const int I = b->prm.solver_iterations;
+  const int nck = (nsub + every - 1) / every;
+  if (a.pk && a.every == every && a.nck >= nck && a.iters == I) return CM_OK;
+  adj_release(b);
+  const size_t n = (size_t)b->n, t = (size_t)b->t, K = (size_t)nck + 1;
+  int rc;
+  if ((rc = adj_alloc(b, &a.ck_x, K * n)) || (rc = adj_alloc(b, &a.ck_v, K * n)) ||
+      (rc = adj_alloc(b, &a.ck_p, K * n)) || (rc = adj_alloc(b, &a.ck_tri, K * t)) ||
+      (rc = adj_alloc(b, &a.pk, (size_t)every * (I + 1) * n)) ||
+      (rc = adj_alloc(b, &a.dpre, (size_t)every * (n + 1))) ||
+      (rc = adj_alloc(b, &a.ctl, 2 + (size_t)every)) || (rc = adj_alloc(b, &a.ab.lx, n)) ||
+      (rc = adj_alloc(b, &a.ab.lv, n)) || (rc = adj_alloc(b, &a.ab.lp, n)) ||
+      (rc = adj_alloc(b, &a.ab.la, n)) || (rc = adj_alloc(b, &a.ab.av, n)) ||
+      (rc = adj_alloc(b, &a.ab.gks, (size_t)b->S)) || (rc = adj_alloc(b, &a.ab.gkb, (size_t)b->B)))
+    return rc;
+  a.arena_cap = (int64_t)every * std::max<int64_t>(b->n, 65536);
+  if ((rc = adj_alloc(b, &a.arena, (size_t)a.arena_cap))) return rc;
+  a.every = every;
+  a.nck = nck;
+  a.iters = I;
+  return CM_OK;
+}
+
+int adj_grow_arena(cm_body* b, int64_t need) {
+  AdjState& a = b->adj;
+  for (size_t k = 0; k < a.allocs.size(); ++k)
+    if (a.allocs[k] == a.arena) {
+      hipFree(a.arena);
+      a.allocs.erase(a.allocs.begin() + k);
+      break;
+    }
+  a.arena = nullptr;
+  a.arena_cap = need + need / 4 + 1024;
+  return adj_alloc(b, &a.arena, (size_t)a.arena_cap);
+}
+
+// checkpoint k <-> the body's substep state: positions, velocities, predicted
+// positions and the triangle order (the sort's tie order); every other buffer is
+// rewritten by the substep before it is read
+int adj_ckpt(cm_body* b, int k, bool save) {
+  AdjState& a = b->adj;
+  const size_t n = (size_t)b->n, t = (size_t)b->t;
+  auto cp = [&](void* ck, void* live, size_t bytes) -> hipError_t {
+    return save ? hipMemcpyAsync(ck, live, bytes, hipMemcpyDeviceToDevice, b->stream)
+                : hipMemcpyAsync(live, ck, bytes, hipMemcpyDeviceToDevice, b->stream);
+  };
+  CM_HIP(cp(a.ck_x + k * n, b->x, sizeof(float4) * n));
+  CM_HIP(cp(a.ck_v + k * n, b->v, sizeof(float4) * n));
+  CM_HIP(cp(a.ck_p + k * n, b->p, sizeof(float4) * n));
+  CM_HIP(cp(a.ck_tri + k * t, b->tri, sizeof(int4) * t));
+  return CM_OK;
+}
+
+int upload_xyz(cm_body* b, const float* src, float4* dst) {
+  std::vector<float4> h(b->n, make_float4(0.f, 0.f, 0.f, 0.f));
+  if (src)
+    for (int i = 0; i < b->n; ++i) h[i] = make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f);
+  CM_HIP(hipMemcpyAsync(dst, h.data(), sizeof(float4) * b->n, hipMemcpyHostToDevice, b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
+  return CM_OK;
+}
+
+// dkp/dk of kp = 1 - (1 - k)^(1 / I)
+double dkp_dk(double k, int iters) {
+  return std::pow(1.0 - k, 1.0 / iters - 1.0) / iters;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_set_stiffness(cm_body* b, const float* k_dist, const float* k_bend) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  HostTopo& h = b->topo;
+  const int I = b->prm.solver_iterations;
+  for (int i = 0; k_dist && i < b->S; ++i)
+    if (!(k_dist[i] >= 0.f && k_dist[i] <= 1.f)) {
+      set_error("cm_set_stiffness: stiffness outside [0, 1]");
+      return CM_ERR_ARG;
+    }
+  for (int i = 0; k_bend && i < b->B; ++i)
+    if (!(k_bend[i] >= 0.f && k_bend[i] <= 1.f)) {
+      set_error("cm_set_stiffness: stiffness outside [0, 1]");
+      return CM_ERR_ARG;
+    }
+  for (int i = 0; k_dist && i < b->S; ++i) {
+    h.k_dist[i] = k_dist[i];
+    h.s_l0kp[i].y = kp_of(k_dist[i], I);
+  }
+  for (int i = 0; k_bend && i < b->B; ++i) {
+    h.k_bend[i] = k_bend[i];
+    h.b_kpphi0[i].x = kp_of(k_bend[i], I);
+  }
+  if (b->ntiles > 0) build_tiles(b->n, h);
+  CM_HIP(hipStreamSynchronize(b->stream));
+  if (b->S)
+    CM_HIP(hipMemcpy(b->s_l0kp, h.s_l0kp.data(), sizeof(float2) * b->S, hipMemcpyHostToDevice));
+  if (b->B)
+    CM_HIP(hipMemcpy(b->b_kpphi0, h.b_kpphi0.data(), sizeof(float2) * b->B,
+                     hipMemcpyHostToDevice));
+  if (b->ntiles > 0)
+    CM_HIP(hipMemcpy(b->entries, h.entries.data(), sizeof(TileEntry) * h.entries.size(),
+                     hipMemcpyHostToDevice));
+  b->adj.ready = false;  // a recorded forward used the old stiffness
+  return CM_OK;
+}
+
+int cm_forward_ckpt(cm_body* b, int32_t substeps, int32_t every) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (substeps < 1 || every < 1) {
+    set_error("cm_forward_ckpt: substeps and every must be positive");
+    return CM_ERR_ARG;
+  }
+  if ((rc = adj_check(b, "cm_forward_ckpt")) || (rc = adj_setup(b, substeps, every)) ||
+      (rc = ensure_graph(b)))
+    return rc;
+  AdjState& a = b->adj;
+  a.ready = false;
+  for (int s = 0; s < substeps; ++s) {
+    if (s % every == 0 && (rc = adj_ckpt(b, s / every, true))) return rc;
+    if (b->exec) {
+      CM_HIP(hipGraphLaunch(b->exec, b->stream));
+    } else if ((rc = record_substep(b))) {
+      return rc;
+    }
+  }
+  if ((rc = adj_ckpt(b, a.nck, true))) return rc;
+  a.nsub = substeps;
+  a.ready = true;
+  return CM_OK;
+}
+
+int cm_adjoint(cm_body* b, const float* dl_dx, const float* dl_dv, float* dl_dx0, float* dl_dv0,
+               float* dl_dk_dist, float* dl_dk_bend) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  AdjState& a = b->adj;
+  if (!a.ready) {
+    set_error("cm_adjoint: no forward on record (cm_forward_ckpt first)");
+    return CM_ERR_STATE;
+  }
+  if (!dl_dx) {
+    set_error("cm_adjoint: dl_dx is required");
+    return CM_ERR_ARG;
+  }
+  const int n = b->n, I = a.iters;
+  const bool coll = (b->prm.flags & CM_F_SELF_COLLISION) && b->t >= 2;
+  if ((rc = upload_xyz(b, dl_dx, a.ab.lx)) || (rc = upload_xyz(b, dl_dv, a.ab.lv))) return rc;
+  CM_HIP(hipMemsetAsync(a.ab.gks, 0, sizeof(double) * std::max(b->S, 1), b->stream));
+  CM_HIP(hipMemsetAsync(a.ab.gkb, 0, sizeof(double) * std::max(b->B, 1), b->stream));
+  const PbdBufs pb = pbd_bufs(b);
+  const PbdParams pp = pbd_params(b);
+  std::vector<int> ctl(2 + a.every);
+  for (int seg = a.nck - 1; seg >= 0; --seg) {
+    const int s0 = seg * a.every, len = std::min(a.every, a.nsub - s0);
+    for (int attempt = 0;; ++attempt) {
+      if ((rc = adj_ckpt(b, seg, false))) return rc;
+      CM_HIP(hipMemsetAsync(a.ctl, 0, sizeof(int) * (2 + a.every), b->stream));
+      for (int j = 0; j < len; ++j) {
+        const TapeSlot ts{a.pk + (int64_t)j * (I + 1) * n, a.dpre + (int64_t)j * (n + 1), j};
+        if ((rc = record_substep(b, nullptr, &ts))) return rc;
+      }
+      if (!coll) break;
+      CM_HIP(hipMemcpyAsync(ctl.data(), a.ctl, sizeof(int) * (2 + len), hipMemcpyDeviceToHost,
+                            b->stream));
+      CM_HIP(hipStreamSynchronize(b->stream));
+      if (!ctl[1]) break;
+      if (attempt > 0) {
+        set_error("cm_adjoint: contact tape overflow");
+        return CM_ERR_ALLOC;
+      }
+      if ((rc = adj_grow_arena(b, ctl[0]))) return rc;
+    }
+    for (int j = len - 1; j >= 0; --j) {
+      const float4* pk = a.pk + (int64_t)j * (I + 1) * n;
+      rc = launch_adj_substep(pb, pp, a.ab, pk, coll ? a.dpre + (int64_t)j * (n + 1) : nullptr,
+                              coll ? a.arena + ctl[2 + j] : nullptr, b->stream);
+      if (rc) return rc;
+    }
+  }
+  if ((rc = adj_ckpt(b, a.nck, false))) return rc;  // back to the state after the forward
+  std::vector<float4> hx(n), hv(n);
+  std::vector<double> gs(b->S), gb(b->B);
+  CM_HIP(hipMemcpyAsync(hx.data(), a.ab.lx, sizeof(float4) * n, hipMemcpyDeviceToHost, b->stream));
+  CM_HIP(hipMemcpyAsync(hv.data(), a.ab.lv, sizeof(float4) * n, hipMemcpyDeviceToHost, b->stream));
+  if (b->S)
+    CM_HIP(hipMemcpyAsync(gs.data(), a.ab.gks, sizeof(double) * b->S, hipMemcpyDeviceToHost,
+                          b->stream));
+  if (b->B)
+    CM_HIP(hipMemcpyAsync(gb.data(), a.ab.gkb, sizeof(double) * b->B, hipMemcpyDeviceToHost,
+                          b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i) {
+    if (dl_dx0) {
+      dl_dx0[3 * i] = hx[i].x;
+      dl_dx0[3 * i + 1] = hx[i].y;
+      dl_dx0[3 * i + 2] = hx[i].z;
+    }
+    if (dl_dv0) {
+      dl_dv0[3 * i] = hv[i].x;
+      dl_dv0[3 * i + 1] = hv[i].y;
+      dl_dv0[3 * i + 2] = hv[i].z;
+    }
+  }
+  for (int i = 0; dl_dk_dist && i < b->S; ++i)
+    dl_dk_dist[i] = (float)(gs[i] * dkp_dk(b->topo.k_dist[i], I));
+  for (int i = 0; dl_dk_bend && i < b->B; ++i)
+    dl_dk_bend[i] = (float)(gb[i] * dkp_dk(b->topo.k_bend[i], I));
+  return CM_OK;
 }
 
 }  // extern "C"

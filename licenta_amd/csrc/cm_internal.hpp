@@ -209,5 +209,29 @@ int launch_solver(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, h
 int launch_solve_only(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, int iters,
                       hipStream_t s);
 int launch_integrate(const PbdBufs& b, const PbdParams& pp, hipStream_t s);
+int launch_integrate_from(const PbdBufs& b, const PbdParams& pp, const float4* pfin,
+                          hipStream_t s);
+// the solver stage with every iterate kept: pk[k * n, (k + 1) * n) = positions
+// entering iteration k, pk[iters * n ...) the result
+int launch_solver_rec(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, float4* pk,
+                      hipStream_t s);
+
+// ---- adjoint (k_pbd.hip) ---------------------------------------------------
+struct AdjBufs {
+  float4 *lx = nullptr, *lv = nullptr;  // dL/dx, dL/dv (n)
+  float4 *lp = nullptr, *la = nullptr;  // dL/dp of the iterates (n each, alternating)
+  float4* av = nullptr;                 // lp / count per vertex (n)
+  double *gks = nullptr, *gkb = nullptr;  // dL/dkp per distance / bend constraint
+  const int* ecid = nullptr;              // per tile entry: constraint index (bit 31: bend)
+};
+// The contact set of substep `slot`: the per-vertex records (dprefix[n] of them) are
+// appended to arena; ctl[0] = records used, ctl[1] = overflow, ctl[2 + slot] = first.
+int launch_tape_records(int n, const int* dprefix, const RecPayload* rec, RecPayload* arena,
+                        int64_t cap, int* ctl, int slot, hipStream_t s);
+// One substep backwards: (lx, lv) = dL/d(x, v) after the substep in, before it out;
+// dL/dkp added to gks / gkb. pk: the substep's iterates (launch_solver_rec);
+// dprefix / rec: its contact records (null: none).
+int launch_adj_substep(const PbdBufs& b, const PbdParams& pp, const AdjBufs& ab, const float4* pk,
+                       const int* dprefix, const RecPayload* rec, hipStream_t s);
 
 }  // namespace cm

@@ -691,6 +691,339 @@ __global__ void __launch_bounds__(kBlock) k_integrate(int n, const float4* __res
   p[i] = pi;
 }
 
+
+// ---- adjoint of the differentiable substep (no reference counterpart) ------------
+// The substep as a function of (x, v, kp): v' = v + dt g, p0 = x + dt v', I Jacobi
+// iterations p_{k+1} = p_k + (sum of the corrections at p_k) / count_k over the
+// distance and bend constraints and the substep's frozen VF / EE contacts, then
+// x' = p_I, v' = (p_I - x) / dt for free vertices (pinned: x' = x, v' = 0). Counts
+// are piecewise constant. Cotangents run in FP32; stiffness gradients sum in FP64.
+
+// x' = p_I, v' = (p_I - x) / dt; pinned: x' = x, v' = 0
+__global__ void __launch_bounds__(kBlock) k_adj_integrate(int n, const uint8_t* __restrict__ pinned,
+                                                          float4* __restrict__ lx,
+                                                          const float4* __restrict__ lv,
+                                                          float4* __restrict__ lp, float dt) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (pinned[i]) {
+    lp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const f3 gv = divs(xyz(lv[i]), dt);
+  lp[i] = f4(add(xyz(lx[i]), gv), 0.f);
+  lx[i] = f4(neg(gv), 0.f);
+}
+
+// p0 = x + dt v', v' = v + dt g
+__global__ void __launch_bounds__(kBlock) k_adj_predict(int n, float4* __restrict__ lx,
+                                                        float4* __restrict__ lv,
+                                                        const float4* __restrict__ lp, float dt) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const f3 g = xyz(lp[i]);
+  lx[i] = f4(add(xyz(lx[i]), g), 0.f);
+  lv[i] = f4(mul(g, dt), 0.f);
+}
+
+// per vertex: the iteration's count at positions ps (static slots plus the dynamic
+// records whose correction is non-zero, as k_solve counts them) and av = lp / count
+__global__ void __launch_bounds__(kBlock) k_adj_scale(int n, const int* __restrict__ vprefix,
+                                                      const int* __restrict__ dprefix,
+                                                      const RecPayload* __restrict__ rec,
+                                                      const float4* __restrict__ ps,
+                                                      const float4* __restrict__ lp,
+                                                      float4* __restrict__ av, float th) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  int cnt = vprefix[i + 1] - vprefix[i];
+  if (dprefix) {
+    const int e = dprefix[i + 1];
+    for (int k = dprefix[i]; k < e; ++k) {
+      const RecPayload rp = rec[k];
+      const float4 P[4] = {ps[rp.ids.x & 0x1fffffff], ps[rp.ids.y], ps[rp.ids.z], ps[rp.ids.w]};
+      if (!is_zero(rec_corr(rp, P, th))) ++cnt;
+    }
+  }
+  av[i] = cnt ? f4(divs(xyz(lp[i]), (float)cnt), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Distance constraint: c0 = -w1 t, c1 = w2 t with t = f kp / ws, f = (l - l0) n.
+// g0 / g1: cotangents of c0 / c1. Returns the cotangent of p_a (p_b gets its
+// negative); gkp receives dL/dkp.
+__device__ __forceinline__ f3 stretch_vjp(float4 A, float4 B, float4 g0, float4 g1, float l0,
+                                          float kp, float& gkp) {
+  gkp = 0.f;
+  const float w1 = A.w, w2 = B.w, ws = w1 + w2;
+  if (ws == 0) return mk3(0, 0, 0);
+  const f3 d = sub(xyz(A), xyz(B));
+  const float l = len(d);
+  if (l <= 1e-8f) return mk3(0, 0, 0);
+  const f3 nn = divs(d, l);
+  const f3 g = sub(mul(xyz(g1), w2), mul(xyz(g0), w1));
+  gkp = dot(g, mul(nn, l - l0)) / ws;
+  const f3 gf = mul(g, kp / ws);
+  // df/dd = I - (l0 / l)(I - n n^T)
+  return sub(gf, mul(sub(gf, mul(nn, dot(nn, gf))), l0 / l));
+}
+
+// Bend constraint (the dihedral correction of bend_all), reverse mode in T: out[k]
+// receives the cotangent of vertex k's position, gkp dL/dkp. The branches whose
+// output is constant (degenerate triangles, d = +-1) give zero.
+template <class T>
+__device__ void bend_vjp(const float4 (&P)[4], const float4 (&G)[4], float kpf, float phi0f,
+                         int abs_mode, f3 out[4], float& gkp) {
+  out[0] = out[1] = out[2] = out[3] = mk3(0, 0, 0);
+  gkp = 0.f;
+  typedef tv<T> V;
+  const T wv[4] = {(T)P[0].w, (T)P[1].w, (T)P[2].w, (T)P[3].w};
+  V P2 = tfrom<T>(P[1]), P3 = tfrom<T>(P[2]), P4 = tfrom<T>(P[3]);
+  if (!abs_mode) {
+    const V p1 = tfrom<T>(P[0]);
+    P2 = tsub(P2, p1);
+    P3 = tsub(P3, p1);
+    P4 = tsub(P4, p1);
+  }
+  const T kp = (T)kpf, phi0 = (T)phi0f;
+  const V c23 = tcross(P2, P3), c24 = tcross(P2, P4);
+  const T l23 = tlen(c23), l24 = tlen(c24);
+  const T eps = (T)1e-8f;
+  if (l23 <= eps || l24 <= eps) return;
+  const V n1 = bdiv<T>(c23, l23), n2 = bdiv<T>(c24, l24);
+  const T d = fmin(fmax((T)-1, tdot(n1, n2)), (T)1);
+  if (d == (T)-1 || d == (T)1) return;
+  const T phi = tacos<T>(d);
+  const V A = tadd(tcross(P3, n2), tmul(tcross(n1, P3), d));
+  const V B = tadd(tcross(P4, n1), tmul(tcross(n2, P4), d));
+  const V C = tadd(tcross(P2, n2), tmul(tcross(n1, P2), d));
+  const V D = tadd(tcross(P2, n1), tmul(tcross(n2, P2), d));
+  const T i23 = (T)1 / l23, i24 = (T)1 / l24;
+  V q[4];
+  q[2] = tmul(C, i23);
+  q[3] = tmul(D, i24);
+  q[1] = tsub(tneg(tmul(A, i23)), tmul(B, i24));
+  q[0] = tsub(tsub(tneg(q[1]), q[2]), q[3]);
+  const T r = tsqrt<T>((T)1 - d * d), e = phi - phi0;
+  const T term = r * e * kp;
+  T sf = 0;
+  for (int k = 0; k < 4; ++k) sf += wv[k] * tdot(q[k], q[k]);
+  if (sf <= eps) return;
+  const T ts = term / sf;
+  // c_k = -w_k ts q_k
+  V qb[4];
+  T tsb = 0;
+  for (int k = 0; k < 4; ++k) {
+    const V ak = tfrom<T>(G[k]);
+    qb[k] = tmul(ak, -wv[k] * ts);
+    tsb -= wv[k] * tdot(q[k], ak);
+  }
+  const T termb = tsb / sf, sfb = -tsb * term / (sf * sf);
+  for (int k = 0; k < 4; ++k) qb[k] = tadd(qb[k], tmul(q[k], (T)2 * wv[k] * sfb));
+  gkp = (float)(termb * r * e);
+  const T rb = termb * e * kp, eb = termb * r * kp;
+  T db = -eb / r - rb * d / r;  // phi = acos(d), r = sqrt(1 - d^2)
+  // q1 = -q2 - q3 - q4
+  qb[1] = tsub(qb[1], qb[0]);
+  qb[2] = tsub(qb[2], qb[0]);
+  qb[3] = tsub(qb[3], qb[0]);
+  // q2 = -A / l23 - B / l24, q3 = C / l23, q4 = D / l24
+  const V Ab = tmul(qb[1], -i23), Bb = tmul(qb[1], -i24);
+  const V Cb = tmul(qb[2], i23), Db = tmul(qb[3], i24);
+  T l23b = (tdot(qb[1], A) - tdot(qb[2], C)) * i23 * i23;
+  T l24b = (tdot(qb[1], B) - tdot(qb[3], D)) * i24 * i24;
+  V P2b = {0, 0, 0}, P3b = {0, 0, 0}, P4b = {0, 0, 0}, n1b = {0, 0, 0}, n2b = {0, 0, 0};
+  // X = U x N + (M x U) d for (U, N, M) = (P3, n2, n1), (P4, n1, n2), (P2, n2, n1), (P2, n1, n2)
+  auto back = [&](const V& Xb, const V& U, const V& N, const V& M, V& Ub, V& Nb, V& Mb) {
+    Ub = tadd(Ub, tcross(N, Xb));
+    Nb = tadd(Nb, tcross(Xb, U));
+    const V t = tmul(Xb, d);
+    Mb = tadd(Mb, tcross(U, t));
+    Ub = tadd(Ub, tcross(t, M));
+    db += tdot(Xb, tcross(M, U));
+  };
+  back(Ab, P3, n2, n1, P3b, n2b, n1b);
+  back(Bb, P4, n1, n2, P4b, n1b, n2b);
+  back(Cb, P2, n2, n1, P2b, n2b, n1b);
+  back(Db, P2, n1, n2, P2b, n1b, n2b);
+  // d = n1 . n2 (interior of the clamp)
+  n1b = tadd(n1b, tmul(n2, db));
+  n2b = tadd(n2b, tmul(n1, db));
+  // n = c / l, l = |c|
+  l23b -= tdot(n1b, c23) * i23 * i23;
+  l24b -= tdot(n2b, c24) * i24 * i24;
+  const V c23b = tadd(tmul(n1b, i23), tmul(n1, l23b));
+  const V c24b = tadd(tmul(n2b, i24), tmul(n2, l24b));
+  // c23 = P2 x P3, c24 = P2 x P4
+  P2b = tadd(P2b, tadd(tcross(P3, c23b), tcross(P4, c24b)));
+  P3b = tadd(P3b, tcross(c23b, P2));
+  P4b = tadd(P4b, tcross(c24b, P2));
+  out[1] = tto(P2b);
+  out[2] = tto(P3b);
+  out[3] = tto(P4b);
+  if (!abs_mode) out[0] = tto(tneg(tadd(tadd(P2b, P3b), P4b)));
+}
+
+// One record of a frozen contact: c_r = k_r g(nn) for the contact's roles r, with
+// nn = sum_r m_r p_r and g(nn) = nn - th nn / |nn| while |nn| < th (fixed weights).
+// Roles whose correction is zero at P were neither summed nor counted, so their
+// cotangents are left out. Returns the cotangent of this record's vertex.
+__device__ __forceinline__ f3 rec_vjp(const RecPayload& rp, const float4 (&P)[4],
+                                      const float4 (&G)[4], float th) {
+  const int role = (rp.ids.x >> 29) & 3;
+  const bool isvf = rp.ids.x < 0;
+  const float4 w = rp.w;
+  bool used[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    used[r] = !is_zero(isvf ? vf_corr(P[0], P[1], P[2], P[3], w, r, th)
+                            : ee_corr(P[0], P[1], P[2], P[3], w, r, th));
+  if (!(used[0] || used[1] || used[2] || used[3])) return mk3(0, 0, 0);
+  float m[4], kr[4];
+  if (isvf) {
+    const float b0 = fclamp(-w.y, 0.f, 1.f), b1 = fclamp(-w.z, 0.f, 1.f),
+                b2 = fclamp(-w.w, 0.f, 1.f);
+    m[0] = 1.f; m[1] = -b0; m[2] = -b1; m[3] = -b2;
+    const float S = P[0].w + P[1].w * b0 * b0 + P[2].w * b1 * b1 + P[3].w * b2 * b2;
+    kr[0] = -P[0].w / S; kr[1] = P[1].w * b0 / S; kr[2] = P[2].w * b1 / S;
+    kr[3] = P[3].w * b2 / S;
+  } else {
+    const float b0 = w.x, b1 = w.y, b2 = -w.z, b3 = -w.w;
+    m[0] = b0; m[1] = b1; m[2] = -b2; m[3] = -b3;
+    const float S = P[0].w * b0 * b0 + P[1].w * b1 * b1 + P[2].w * b2 * b2 + P[3].w * b3 * b3;
+    kr[0] = -P[0].w * b0 / S; kr[1] = -P[1].w * b1 / S; kr[2] = P[2].w * b2 / S;
+    kr[3] = P[3].w * b3 / S;
+  }
+  f3 nn = mk3(0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) nn = add(nn, mul(xyz(P[r]), m[r]));
+  const float l = len(nn);
+  const f3 nh = divs(nn, l);
+  f3 g = mk3(0, 0, 0);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (used[r]) g = add(g, mul(xyz(G[r]), kr[r]));
+  // dg/dnn = I - (th / l)(I - n n^T)
+  const f3 gn = sub(g, mul(sub(g, mul(nh, dot(nh, g))), th / l));
+  return mul(gn, m[role]);
+}
+
+struct AdjArgs {
+  int abs_mode, bend_fp64;
+  float th;
+  const int* vprefix;
+  const int* tstart;
+  const int* eprefix;
+  const TileEntry* entries;
+  const int* ecid;  // per entry: constraint index, bit 31 set for a bend
+  const int* dprefix;
+  const RecPayload* rec;
+  double* gks;  // dL/dkp per distance constraint
+  double* gkb;  // dL/dkp per bend constraint
+};
+
+// One reverse Jacobi iteration over a vertex tile, in the layout of k_solve<true>:
+// every constraint touching the tile is differentiated once, its vertex cotangents
+// go to the tile's slots, and each vertex adds its slots in slot order, then its
+// dynamic records in record order: la = lp + sum over slots and records. The tile
+// holding a constraint's role-0 vertex adds its dL/dkp (one owner per constraint).
+__global__ void __launch_bounds__(kBlock) k_adj_iter(AdjArgs a, const float4* __restrict__ ps,
+                                                     const float4* __restrict__ av,
+                                                     const float4* __restrict__ lp,
+                                                     float4* __restrict__ la) {
+  __shared__ float cx[kLds], cy[kLds], cz[kLds];
+  const int v0 = a.tstart[blockIdx.x], v1 = a.tstart[blockIdx.x + 1];
+  const int tid = threadIdx.x;
+  const int i = v0 + tid;
+  const bool mine = i < v1;
+  f3 total = mk3(0, 0, 0);
+  {
+    const int s0 = a.vprefix[v0];
+    const int eb = a.eprefix[blockIdx.x], ee = a.eprefix[blockIdx.x + 1];
+    for (int q = eb + tid; q < ee; q += kBlock) {
+      const TileEntry t = a.entries[q];
+      const int cid = a.ecid[q];
+      float gkp;
+      if (t.ids.z < 0) {
+        const float4 P0 = ps[t.ids.x], P1 = ps[t.ids.y];
+        const float4 G0 = av[t.ids.x], G1 = av[t.ids.y];
+        const f3 gd = stretch_vjp(P0, P1, G0, G1, t.a, t.b, gkp);
+        if ((t.s01 & 0xffffu) != 0xffffu) {
+          lds_put(cx, cy, cz, t.s01 & 0xffffu, gd);
+          a.gks[cid] += (double)gkp;
+        }
+        if ((t.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, t.s01 >> 16, neg(gd));
+      } else {
+        const float4 P[4] = {ps[t.ids.x], ps[t.ids.y], ps[t.ids.z], ps[t.ids.w]};
+        const float4 G[4] = {av[t.ids.x], av[t.ids.y], av[t.ids.z], av[t.ids.w]};
+        f3 o[4];
+        if (a.bend_fp64)
+          bend_vjp<double>(P, G, t.a, t.b, a.abs_mode, o, gkp);
+        else
+          bend_vjp<float>(P, G, t.a, t.b, a.abs_mode, o, gkp);
+        if ((t.s01 & 0xffffu) != 0xffffu) {
+          lds_put(cx, cy, cz, t.s01 & 0xffffu, o[0]);
+          a.gkb[cid & 0x7fffffff] += (double)gkp;
+        }
+        if ((t.s01 >> 16) != 0xffffu) lds_put(cx, cy, cz, t.s01 >> 16, o[1]);
+        if ((t.s23 & 0xffffu) != 0xffffu) lds_put(cx, cy, cz, t.s23 & 0xffffu, o[2]);
+        if ((t.s23 >> 16) != 0xffffu) lds_put(cx, cy, cz, t.s23 >> 16, o[3]);
+      }
+    }
+    __syncthreads();
+    if (mine) {
+      const int e = a.vprefix[i + 1];
+      for (int k = a.vprefix[i]; k < e; ++k)
+        total = add(total, mk3(cx[k - s0], cy[k - s0], cz[k - s0]));
+    }
+    __syncthreads();
+  }
+  if (a.dprefix) {
+    const int r0 = a.dprefix[v0], r1 = a.dprefix[v1];
+    int k = mine ? a.dprefix[i] : 0;
+    const int e = mine ? a.dprefix[i + 1] : 0;
+    for (int c0 = r0; c0 < r1; c0 += kLds) {
+      const int c1 = min(c0 + kLds, r1);
+      for (int q = c0 + tid; q < c1; q += kBlock) {
+        const RecPayload rp = a.rec[q];
+        const int id0 = rp.ids.x & 0x1fffffff;
+        const float4 P[4] = {ps[id0], ps[rp.ids.y], ps[rp.ids.z], ps[rp.ids.w]};
+        const float4 G[4] = {av[id0], av[rp.ids.y], av[rp.ids.z], av[rp.ids.w]};
+        lds_put(cx, cy, cz, q - c0, rec_vjp(rp, P, G, a.th));
+      }
+      __syncthreads();
+      const int kend = min(e, c1);
+      for (; k < kend; ++k) total = add(total, mk3(cx[k - c0], cy[k - c0], cz[k - c0]));
+      __syncthreads();
+    }
+  }
+  if (mine) la[i] = f4(add(xyz(lp[i]), total), 0.f);
+}
+
+// Tape of a recorded substep's contact set: slot `slot` of ctl ([0] records used, [1]
+// overflow flag, [2 + slot] the slot's first record or -1) takes the next dprefix[n]
+// records of the arena. The cursor advances even past the capacity, so the host
+// learns what a segment needs.
+__global__ void k_tape_claim(int n, const int* __restrict__ dprefix, int64_t cap,
+                             int* __restrict__ ctl, int slot) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int off = ctl[0], nrec = dprefix[n];
+  const bool fits = (int64_t)off + nrec <= cap;
+  ctl[2 + slot] = fits ? off : -1;
+  if (!fits) ctl[1] = 1;
+  ctl[0] = off + nrec;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tape_copy(int n, const int* __restrict__ dprefix,
+                                                      const RecPayload* __restrict__ rec,
+                                                      RecPayload* __restrict__ arena,
+                                                      const int* __restrict__ ctl, int slot) {
+  const int off = ctl[2 + slot];
+  if (off < 0) return;
+  const int nrec = dprefix[n];
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nrec; k += gridDim.x * kBlock)
+    arena[(int64_t)off + k] = rec[k];
+}
+
 }  // namespace
 
 int launch_predict(const PbdBufs& b, const PbdParams& pp, hipStream_t s) {
@@ -774,10 +1107,84 @@ int launch_solve_only(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& n
 }
 
 int launch_integrate(const PbdBufs& b, const PbdParams& pp, hipStream_t s) {
-  const float4* pfin = (pp.iters & 1) ? b.p2 : b.p;
+  return launch_integrate_from(b, pp, (pp.iters & 1) ? b.p2 : b.p, s);
+}
+
+int launch_integrate_from(const PbdBufs& b, const PbdParams& pp, const float4* pfin,
+                          hipStream_t s) {
   hipLaunchKernelGGL(k_integrate, dim3(nblocks(b.n)), dim3(kBlock), 0, s, b.n, pfin, b.x, b.v,
                      b.p, b.pinned, pp.dt, (pp.flags & CM_F_WORLD_CLAMP) ? 1 : 0, pp.wmin,
                      pp.wmax);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_solver_rec(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, float4* pk,
+                      hipStream_t s) {
+  const int g = nblocks(b.n);
+  const int ground_on = (pp.flags & CM_F_GROUND) ? 1 : 0;
+  const int F = (pp.flags & CM_F_LRA) ? b.F : 0;
+  hipLaunchKernelGGL(k_setup, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.gh, pp.th, ground_on,
+                     b.ground, F, b.lra_fixed, b.lra_l0, b.pinned, b.lra_flag);
+  if (pp.flags & CM_F_SPHERE)
+    hipLaunchKernelGGL(k_sphere, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.sc, pp.sr + pp.th);
+  CM_LAUNCH_CHECK();
+  CM_HIP(hipMemcpyAsync(pk, b.p, sizeof(float4) * b.n, hipMemcpyDeviceToDevice, s));
+  const SolveArgs a = solve_args(b, pp, nb);
+  for (int it = 0; it < pp.iters; ++it) {
+    const float4* src = pk + (int64_t)it * b.n;
+    float4* dst = pk + (int64_t)(it + 1) * b.n;
+    if (b.ntiles > 0)
+      hipLaunchKernelGGL(k_solve<true>, dim3(b.ntiles), dim3(kBlock), 0, s, a, src, dst);
+    else
+      hipLaunchKernelGGL(k_solve<false>, dim3((b.n + kSolveV - 1) / kSolveV), dim3(kBlock), 0, s,
+                         a, src, dst);
+  }
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_tape_records(int n, const int* dprefix, const RecPayload* rec, RecPayload* arena,
+                        int64_t cap, int* ctl, int slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_tape_claim, dim3(1), dim3(64), 0, s, n, dprefix, cap, ctl, slot);
+  hipLaunchKernelGGL(k_tape_copy, dim3(1024), dim3(kBlock), 0, s, n, dprefix, rec, arena, ctl,
+                     slot);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_adj_substep(const PbdBufs& b, const PbdParams& pp, const AdjBufs& ab, const float4* pk,
+                       const int* dprefix, const RecPayload* rec, hipStream_t s) {
+  if (b.ntiles <= 0) {
+    set_error("adjoint: needs the tiled Jacobi layout");
+    return CM_ERR_ARG;
+  }
+  const int g = nblocks(b.n);
+  hipLaunchKernelGGL(k_adj_integrate, dim3(g), dim3(kBlock), 0, s, b.n, b.pinned, ab.lx, ab.lv,
+                     ab.lp, pp.dt);
+  AdjArgs a;
+  a.abs_mode = (pp.ref_compat & CM_RC_BEND_ABS) ? 1 : 0;
+  a.bend_fp64 = (pp.flags & CM_F_BEND_FP64) ? 1 : 0;
+  a.th = pp.th;
+  a.vprefix = b.vprefix;
+  a.tstart = b.tstart;
+  a.eprefix = b.eprefix;
+  a.entries = b.entries;
+  a.ecid = ab.ecid;
+  a.dprefix = dprefix;
+  a.rec = rec;
+  a.gks = ab.gks;
+  a.gkb = ab.gkb;
+  float4* src = ab.lp;
+  float4* dst = ab.la;
+  for (int it = pp.iters - 1; it >= 0; --it) {
+    const float4* ps = pk + (int64_t)it * b.n;
+    hipLaunchKernelGGL(k_adj_scale, dim3(g), dim3(kBlock), 0, s, b.n, b.vprefix, dprefix, rec, ps,
+                       src, ab.av, pp.th);
+    hipLaunchKernelGGL(k_adj_iter, dim3(b.ntiles), dim3(kBlock), 0, s, a, ps, ab.av, src, dst);
+    std::swap(src, dst);
+  }
+  hipLaunchKernelGGL(k_adj_predict, dim3(g), dim3(kBlock), 0, s, b.n, ab.lx, ab.lv, src, pp.dt);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
