@@ -63,6 +63,12 @@ def main():
     ap.add_argument("--batch-cloths", type=int, default=512,
                     help="cloths of the batched-vs-plain leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--adj-size", type=int, default=128,
+                    help="side of the fitting cloth of the adjoint leg (0 skips it)")
+    ap.add_argument("--adj-substeps", type=int, default=60)
+    ap.add_argument("--adj-every", type=int, default=10)
+    ap.add_argument("--adj-batch", type=int, default=64,
+                    help="64x64 cloths of the batched adjoint leg (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -144,6 +150,10 @@ def main():
     body.close()
     out["batched"] = batched_leg(args.batch_cloths, args.batch_steps) if args.batch_cloths > 0 \
         else None
+    out["adjoint"] = adjoint_leg(args.adj_size, args.adj_substeps, args.adj_every) \
+        if args.adj_size > 0 else None
+    out["batched_adjoint"] = batched_adjoint_leg(args.adj_batch, 30, 10) \
+        if args.adj_batch > 0 else None
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(configs.config_scene(1, args.size), args.cpu_budget)
         out["cpu_baseline"]["cpu_model"] = _cpu_model()
@@ -192,6 +202,90 @@ def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
                 batched_over_plain=batched / plain, batched_ms_per_step=tb * 1e3 / steps,
                 plain_ms_per_step=tp * 1e3 / steps,
                 last_substep=dict(pairs=bcounts["pairs"], vf=bcounts["vf"], ee=bcounts["ee"]))
+
+
+def _time_rollout(body, substeps, every, x0, v0, tri0, lx_of, reps=3):
+    """Median wall times of: plain forward (cm_step), checkpointed forward, adjoint."""
+    plain, fwd, adj = [], [], []
+    for rep in range(reps + 1):  # the first round warms up (graph capture, buffers)
+        body.set_state(x0, v0, tri=tri0)
+        body.sync()
+        t0 = time.perf_counter()
+        body.fixed_update(substeps)
+        body.sync()
+        t1 = time.perf_counter()
+        body.set_state(x0, v0, tri=tri0)
+        body.sync()
+        t2 = time.perf_counter()
+        body.forward_ckpt(substeps, every)
+        body.sync()
+        t3 = time.perf_counter()
+        lx = lx_of(body.positions())
+        t4 = time.perf_counter()
+        grads = body.adjoint(lx)
+        t5 = time.perf_counter()
+        if rep:
+            plain.append(t1 - t0)
+            fwd.append(t3 - t2)
+            adj.append(t5 - t4)
+    med = lambda a: float(np.median(a))
+    return med(plain), med(fwd), med(adj), grads
+
+
+def adjoint_leg(size: int, substeps: int, every: int) -> dict:
+    """Adjoint/forward time ratio of the fitting workload: a 128x128 pinned cloth,
+    60 substeps, 10 iterations, self-collision, from the fit's start (stiffness 0.65
+    everywhere, zero velocity), loss = mean squared vertex error against the
+    ground-truth run's final positions, checkpoints every 10 substeps."""
+    from licenta_amd import configs
+    from licenta_amd.body import ClothBody
+
+    sc = configs.config_scene(3, size)
+    body = ClothBody.from_scene(sc)  # ground-truth stiffness and velocities
+    tri0 = body.buffer("tri").copy()
+    body.set_state(sc["xyz"], sc["v0"], tri=tri0)
+    body.fixed_update(substeps)
+    target = body.positions().astype(np.float64)
+    c = body.counts()
+    body.set_stiffness(np.full(c["s"], 0.65, np.float32))
+    zero = np.zeros_like(sc["xyz"])
+    n = len(target)
+    plain, fwd, adj, grads = _time_rollout(
+        body, substeps, every, sc["xyz"], zero, tri0,
+        lambda x: (2.0 * (x - target) / n).astype(np.float32))
+    counts = body.counts()
+    body.close()
+    return dict(workload=f"fit_{size}x{size}: {substeps} substeps, checkpoint every {every}",
+                vertices=n, distance_constraints=c["s"], bend_constraints=c["b"],
+                forward_ms=plain * 1e3, forward_ckpt_ms=fwd * 1e3, adjoint_ms=adj * 1e3,
+                adjoint_over_forward=adj / plain,
+                forward_plus_adjoint_over_forward=(fwd + adj) / plain,
+                last_substep_contacts=counts["vf"] + counts["ee"],
+                grad_norm_k_dist=float(np.linalg.norm(grads[2])))
+
+
+def batched_adjoint_leg(ncloth: int, substeps: int, every: int) -> dict:
+    """Forward + adjoint of 64 cloths of 64x64 in one batched body (per-cloth bend
+    stiffness), 30 substeps, checkpoints every 10; loss against the run's own
+    final positions shifted by 1 mm."""
+    from licenta_amd import configs
+    from licenta_amd.body import ClothBody
+
+    scenes = configs.batched_scene(4, count=ncloth)
+    body = ClothBody.batch(scenes)
+    x0 = np.concatenate([s["xyz"] for s in scenes])
+    tri0 = body.buffer("tri").copy()
+    zero = np.zeros_like(x0)
+    n = len(x0)
+    plain, fwd, adj, _ = _time_rollout(
+        body, substeps, every, x0, zero, tri0,
+        lambda x: (2.0 * (x - (x + 1e-3)) / n).astype(np.float32))
+    body.close()
+    return dict(workload=f"{ncloth} cloths of 64x64, {substeps} substeps, checkpoint every "
+                         f"{every}", vertices=n, forward_ms=plain * 1e3,
+                forward_ckpt_ms=fwd * 1e3, adjoint_ms=adj * 1e3,
+                adjoint_over_forward=adj / plain,
+                forward_plus_adjoint_over_forward=(fwd + adj) / plain)
 
 
 def calibrate_copy():

@@ -193,3 +193,41 @@ def test_adjoint_rejects():
     with pytest.raises(RuntimeError, match="error -1"):
         body.forward_ckpt(2, 1)
     body.close()
+
+
+def test_batched_adjoint_matches_single_bodies():
+    """Three 64 x 64 cloths of the batched-gradient workload stepped as one body give
+    each cloth the gradients of its own single-cloth body."""
+    from licenta_amd.body import ClothBody
+    print("adjoint batch of 3 x 64^2", flush=True)
+    scenes = configs.batched_scene(4, count=3)
+    substeps, every = 6, 4
+    rng = np.random.default_rng(3)
+    v0s = [rng.normal(0.0, 0.05, size=(len(s["xyz"]), 3)).astype(np.float32) for s in scenes]
+    targets = [s["xyz"] + rng.normal(0.0, 0.01, size=(len(s["xyz"]), 3)) for s in scenes]
+    singles = []
+    for s, v0, tg in zip(scenes, v0s, targets):
+        b = ClothBody.from_scene(s)
+        b.set_stiffness(np.full(b.counts()["s"], 0.9, np.float32))  # dk/dkp is finite below 1
+        b.set_state(s["xyz"], v0)
+        b.forward_ckpt(substeps, every)
+        lx = 2.0 * (b.positions() - tg) / len(tg)
+        singles.append((b.counts(), b.adjoint(lx)))
+        b.close()
+    batch = ClothBody.batch(scenes)
+    batch.set_stiffness(np.full(batch.counts()["s"], 0.9, np.float32))
+    batch.set_state(np.concatenate([s["xyz"] for s in scenes]), np.concatenate(v0s))
+    batch.forward_ckpt(substeps, every)
+    x = batch.positions()
+    n0 = np.cumsum([0] + [len(s["xyz"]) for s in scenes])
+    lx = np.concatenate([2.0 * (x[n0[c]:n0[c + 1]] - targets[c]) / len(targets[c])
+                         for c in range(len(scenes))])
+    gx, gv, gkd, gkb = batch.adjoint(lx)
+    s0 = np.cumsum([0] + [c["s"] for c, _ in singles])
+    b0 = np.cumsum([0] + [c["b"] for c, _ in singles])
+    for c, (_, (sx, sv, skd, skb)) in enumerate(singles):
+        assert scaled_err(gx[n0[c]:n0[c + 1]], sx) <= 1e-5
+        assert scaled_err(gv[n0[c]:n0[c + 1]], sv) <= 1e-5
+        assert scaled_err(gkd[s0[c]:s0[c + 1]], skd) <= 1e-5
+        assert scaled_err(gkb[b0[c]:b0[c + 1]], skb) <= 1e-5
+    batch.close()
