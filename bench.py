@@ -63,6 +63,9 @@ def main():
     ap.add_argument("--batch-cloths", type=int, default=512,
                     help="cloths of the batched-vs-plain leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--large-size", type=int, default=1024,
+                    help="side of the crumpled large cloth (0 skips that leg)")
+    ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--adj-size", type=int, default=128,
                     help="side of the fitting cloth of the adjoint leg (0 skips it)")
     ap.add_argument("--adj-substeps", type=int, default=60)
@@ -150,6 +153,7 @@ def main():
     body.close()
     out["batched"] = batched_leg(args.batch_cloths, args.batch_steps) if args.batch_cloths > 0 \
         else None
+    out["large"] = large_leg(args.large_size, args.large_steps) if args.large_size > 0 else None
     out["adjoint"] = adjoint_leg(args.adj_size, args.adj_substeps, args.adj_every) \
         if args.adj_size > 0 else None
     out["batched_adjoint"] = batched_adjoint_leg(args.adj_batch, 30, 10) \
@@ -202,6 +206,34 @@ def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
                 batched_over_plain=batched / plain, batched_ms_per_step=tb * 1e3 / steps,
                 plain_ms_per_step=tp * 1e3 / steps,
                 last_substep=dict(pairs=bcounts["pairs"], vf=bcounts["vf"], ee=bcounts["ee"]))
+
+
+def large_leg(size: int, steps: int, warmup: int = 3) -> dict:
+    """The crumpled large cloth: 1024x1024 vertices, spacing 1 mm, y displaced by
+    8 random sinusoids (seed 4), thickness 0.5 mm, 20 iterations, 1 substep per
+    frame at dt 1/60 s, 200 frames, full Morton/sort/LBVH rebuild every substep."""
+    from licenta_amd import configs
+    from licenta_amd.body import ClothBody
+
+    sc = configs.config_scene(4, size)
+    t0 = time.perf_counter()
+    body = ClothBody.from_scene(sc)
+    t_create = time.perf_counter() - t0
+    body.fixed_update(warmup)
+    body.sync()
+    t0 = time.perf_counter()
+    body.fixed_update(steps)
+    body.sync()
+    dt = time.perf_counter() - t0
+    c = body.counts()
+    w = body.warnings()
+    body.close()
+    return dict(workload=f"crumpled_{size}x{size}", vertices=c["n"], triangles=c["t"],
+                distance_constraints=c["s"], bend_constraints=c["b"], iterations=20,
+                steps=steps, warmup=warmup, substeps_per_s=steps / dt,
+                vertex_updates_per_s=c["n"] * steps / dt, ms_per_step=dt * 1e3 / steps,
+                create_s=t_create,
+                last_substep=dict(pairs=c["pairs"], vf=c["vf"], ee=c["ee"], warnings=w))
 
 
 def _time_rollout(body, substeps, every, x0, v0, tri0, lx_of, reps=3):

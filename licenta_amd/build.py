@@ -57,9 +57,27 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_wrapper_demo()
     if verbose:
         print(f"built {OUT}")
     return OUT
+
+
+DEMO_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "wrapper_demo.cpp")
+DEMO = os.path.join(os.path.dirname(HERE), "tests", "cpp", "wrapper_demo")
+
+
+def build_wrapper_demo() -> str:
+    """The C++ wrapper's test driver (include/clothmi.hpp), a plain g++ program
+    linked against the library."""
+    rocm_lib = "/opt/rocm/lib"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", DEMO_SRC, "-o", DEMO, f"-L{HERE}", "-lclothmi",
+           f"-L{rocm_lib}", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../licenta_amd",
+           f"-Wl,-rpath,{rocm_lib}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"wrapper demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return DEMO
 
 
 if __name__ == "__main__":

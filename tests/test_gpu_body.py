@@ -110,6 +110,27 @@ def test_sphere_drop_small(oracle_mod):
     res, w = run_scene(oracle_mod, sc, 3)
 
 
+def test_fit_scene_small(oracle_mod):
+    """The fitting workload's forward (pinned corners, per-constraint stiffness,
+    random initial velocities) at 24 x 24."""
+    sc = configs.config_scene(3, 24)
+    from licenta_amd.body import ClothBody
+    body = ClothBody.from_scene(sc)
+    body.set_state(sc["xyz"], sc["v0"])
+    for s in range(3):
+        print(f"[{sc['name']}] substep {s}", flush=True)
+        compare_substep(oracle_mod, sc, body)
+    body.close()
+
+
+def test_crumpled_small(oracle_mod):
+    """The crumpled large-cloth workload (sinusoid-displaced, dense self-contact at
+    thickness 0.5 mm, 20 iterations) at 40 x 40."""
+    sc = configs.config_scene(4, 40)
+    res, w = run_scene(oracle_mod, sc, 3)
+    assert all(r["pairs"] > 0 for r in res)
+
+
 def test_reference_scene_defaults(oracle_mod):
     """Ground, world clamp, damping, LRA and 63-bit raw Morton codes on."""
     sc = configs.reference_scene(24)
