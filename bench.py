@@ -37,18 +37,34 @@ def algorithmic_bytes(n, s, bd, t, iters, contacts, pairs):
 
 
 def cpu_baseline(scene, budget_s: float):
+    """The FP32 oracle (the C++ restatement of the reference path) on this host:
+    all cores (OpenMP over vertices, constraints, triangles, leaves, pairs and
+    candidates; results identical to one thread), then one thread, each on the
+    first substeps of the same workload within the time budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the CPU restatement, used only as the reported baseline
 
-    o = oracle.OracleSim(scene, 32)
-    done, elapsed = 0, 0.0
-    while elapsed < budget_s and done < 64:
-        elapsed += o.time_steps(1)
-        done += 1
-    return dict(value=done / elapsed, unit="substeps/s", cores=1, kind="port",
-                sample=f"first {done} substeps of the same 256x256 sphere-drop workload, "
-                       f"FP32 oracle (C++ restatement of the reference path), 1 thread, "
-                       f"{elapsed:.1f} s")
+    def run(threads, budget):
+        used = oracle.OracleSim.set_threads(threads)
+        o = oracle.OracleSim(scene, 32)
+        done, elapsed = 0, 0.0
+        while elapsed < budget and done < 64:
+            elapsed += o.time_steps(1)
+            done += 1
+        return used, done, elapsed
+
+    default = oracle.OracleSim.set_threads(0)
+    try:
+        cores, done, elapsed = run(default, budget_s)
+        _, done1, elapsed1 = run(1, budget_s / 2)
+    finally:
+        oracle.OracleSim.set_threads(default)
+    return dict(value=done / elapsed, unit="substeps/s", cores=cores, kind="port",
+                sample=f"first {done} substeps of the same 256x256 sphere-drop workload, FP32 "
+                       f"oracle (C++ restatement of the reference path), {cores} OpenMP "
+                       f"threads, {elapsed:.1f} s",
+                single_thread=dict(value=done1 / elapsed1, unit="substeps/s", cores=1,
+                                   sample=f"first {done1} substeps, 1 thread, {elapsed1:.1f} s"))
 
 
 def main():

@@ -38,6 +38,22 @@
 
 #include "../include/clothmi.h"
 
+// Loops over independent items (vertices, constraints, triangles, leaves, pairs,
+// candidates) run on OpenMP threads; every result is written to its own index and
+// joined in index order, so the output does not depend on the thread count.
+#ifdef _OPENMP
+#include <omp.h>
+#define CM_OMP_FOR _Pragma("omp parallel for schedule(static)")
+inline int omp_threads() { return omp_get_max_threads(); }
+inline int omp_tid() { return omp_get_thread_num(); }
+inline int omp_nthreads() { return omp_get_num_threads(); }
+#else
+#define CM_OMP_FOR
+inline int omp_threads() { return 1; }
+inline int omp_tid() { return 0; }
+inline int omp_nthreads() { return 1; }
+#endif
+
 namespace orc {
 
 // ---------------------------------------------------------------------------
@@ -223,14 +239,17 @@ inline void lbvh_build(int t, const uint64_t* keys, bool midpoint, int32_t* left
                        int32_t* rights, int32_t* parents, uint32_t* warn) {
   const int nin = t - 1;
   std::vector<int> claims(2 * (size_t)t - 1, 0);
+  CM_OMP_FOR
   for (int id = 0; id < nin; ++id) {
     int first, last;
     find_range(keys, nin, id, first, last);
     int split = find_split(keys, nin, first, last, midpoint);
-    int l = split == first ? nin + split : split;
-    int r = split + 1 == last ? nin + split + 1 : split + 1;
-    lefts[id] = l;
-    rights[id] = r;
+    lefts[id] = split == first ? nin + split : split;
+    rights[id] = split + 1 == last ? nin + split + 1 : split + 1;
+  }
+  // parents in node order: on a broken tree the last claimant wins, as in one pass
+  for (int id = 0; id < nin; ++id) {
+    const int l = lefts[id], r = rights[id];
     parents[l] = id;
     parents[r] = id;
     if (++claims[l] > 1 || ++claims[r] > 1) {
@@ -978,6 +997,7 @@ struct Sim {
     // stage 2: swept leaf AABBs + keys in current triangle order
     // (Physics/CudaDeformableUtils.cu:120-151)
     std::vector<F3> cen(t);
+    CM_OMP_FOR
     for (int i = 0; i < t; ++i) {
       const Tri& tr = tri[i];
       F3 mn = subs(min3(pf[tr.v1], pf[tr.v2], pf[tr.v3]), th);
@@ -1018,41 +1038,71 @@ struct Sim {
     lbvh_refit(t, lefts.data(), rights.data(), amin.data(), amax.data(), rmll.data(), rmlr.data());
     // stage 4: traversal -> pairs (Collision/CudaBVH.cu:23-113), compaction in leaf order
     const int cap = std::min(prm.pair_cap, t);  // Physics/CudaDeformableBody.cpp:233
-    std::vector<int32_t> buf(2 * (size_t)cap);
-    for (int i = 0; i < t; ++i) {
-      bool capped = false;
-      int c = lbvh_leaf_pairs(t, i, lefts.data(), rights.data(), amin.data(), amax.data(),
-                              rmll.data(), rmlr.data(), cap, buf.data(), &warn, &capped);
-      if (capped) warn |= CM_WARN_PAIR_CAP;
-      pairs.insert(pairs.end(), buf.begin(), buf.begin() + 2 * c);
+    {
+      // per leaf in parallel, concatenated in leaf order
+      std::vector<int32_t> buf((size_t)t * 2 * cap);
+      std::vector<int> cnt(t);
+      uint32_t w = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : w)
+      for (int i = 0; i < t; ++i) {
+        bool capped = false;
+        uint32_t wl = 0;
+        cnt[i] = lbvh_leaf_pairs(t, i, lefts.data(), rights.data(), amin.data(), amax.data(),
+                                 rmll.data(), rmlr.data(), cap, buf.data() + (size_t)i * 2 * cap,
+                                 &wl, &capped);
+        w |= wl | (capped ? (uint32_t)CM_WARN_PAIR_CAP : 0u);
+      }
+      warn |= w;
+      for (int i = 0; i < t; ++i)
+        pairs.insert(pairs.end(), buf.begin() + (size_t)i * 2 * cap,
+                     buf.begin() + (size_t)i * 2 * cap + 2 * cnt[i]);
     }
     // stage 5: candidates, feature filter, CCD, compaction
-    // (Collision/CudaCollisionDetection.cu:6-189, 223-283)
+    // (Collision/CudaCollisionDetection.cu:6-189, 223-283); threads take contiguous
+    // pair ranges and their lists are joined in range order
     const size_t P = pairs.size() / 2;
-    std::vector<cm_contact> vfc, eec;
-    for (size_t k = 0; k < P; ++k) {
-      const Tri& t1 = tri[pairs[2 * k]];
-      const Tri& tt2 = tri[pairs[2 * k + 1]];
-      for (int s = 0; s < 6; ++s) {
-        cm_contact c;
-        if (candidate(t1, tt2, s, c) && feature_filter(c, true, pf.data(), xf.data(), th)) vfc.push_back(c);
-      }
-    }
-    if (prm.flags & CM_F_EE_CONTACTS) {
-      for (size_t k = 0; k < P; ++k) {
+    const bool ee_on = (prm.flags & CM_F_EE_CONTACTS) != 0;
+    const int nth = omp_threads();
+    std::vector<std::vector<cm_contact>> pvf(nth), pee(nth);
+#pragma omp parallel num_threads(nth)
+    {
+      const int id = omp_tid(), nt = omp_nthreads();
+      const size_t lo = P * id / nt, hi = P * (id + 1) / nt;
+      for (size_t k = lo; k < hi; ++k) {
         const Tri& t1 = tri[pairs[2 * k]];
         const Tri& tt2 = tri[pairs[2 * k + 1]];
-        for (int s = 6; s < 15; ++s) {
+        for (int s = 0; s < 6; ++s) {
           cm_contact c;
-          if (candidate(t1, tt2, s, c) && feature_filter(c, false, pf.data(), xf.data(), th)) eec.push_back(c);
+          if (candidate(t1, tt2, s, c) && feature_filter(c, true, pf.data(), xf.data(), th))
+            pvf[id].push_back(c);
         }
       }
+      if (ee_on)
+        for (size_t k = lo; k < hi; ++k) {
+          const Tri& t1 = tri[pairs[2 * k]];
+          const Tri& tt2 = tri[pairs[2 * k + 1]];
+          for (int s = 6; s < 15; ++s) {
+            cm_contact c;
+            if (candidate(t1, tt2, s, c) && feature_filter(c, false, pf.data(), xf.data(), th))
+              pee[id].push_back(c);
+          }
+        }
+    }
+    std::vector<cm_contact> vfc, eec;
+    for (int id = 0; id < nth; ++id) {
+      vfc.insert(vfc.end(), pvf[id].begin(), pvf[id].end());
+      eec.insert(eec.end(), pee[id].begin(), pee[id].end());
     }
     const float dt = prm.timestep;
-    for (auto& c : vfc)
-      if (ccd_vf(c, xf.data(), vel.data(), th, dt)) vf.push_back(c);
-    for (auto& c : eec)
-      if (ccd_ee(c, xf.data(), vel.data(), th, dt)) ee.push_back(c);
+    std::vector<uint8_t> hvf(vfc.size()), hee(eec.size());
+#pragma omp parallel for schedule(dynamic, 512)
+    for (long k = 0; k < (long)vfc.size(); ++k) hvf[k] = ccd_vf(vfc[k], xf.data(), vel.data(), th, dt);
+#pragma omp parallel for schedule(dynamic, 512)
+    for (long k = 0; k < (long)eec.size(); ++k) hee[k] = ccd_ee(eec[k], xf.data(), vel.data(), th, dt);
+    for (size_t k = 0; k < vfc.size(); ++k)
+      if (hvf[k]) vf.push_back(vfc[k]);
+    for (size_t k = 0; k < eec.size(); ++k)
+      if (hee[k]) ee.push_back(eec[k]);
   }
   void vf_clear() {
     vf.clear();
@@ -1071,8 +1121,10 @@ struct Sim {
         lo = F3{0, 0, 0};
         hi = F3{1, 1, 1};
       }
+      CM_OMP_FOR
       for (int i = 0; i < t; ++i) keys[i] = morton30(cen[i], lo, hi);
     } else {
+      CM_OMP_FOR
       for (int i = 0; i < t; ++i) keys[i] = morton63(cen[i]);
     }
   }
@@ -1319,17 +1371,20 @@ struct Sim {
           cnt[i] += c;
         }
       // static constraints into slots, then per-vertex gather (:942-962, :1417-1446)
+      CM_OMP_FOR
       for (int s = 0; s < S; ++s) {
         RV c1, c2;
         stretch(s, p, c1, c2);
         slots[tp.soff[0][s]] = c1;
         slots[tp.soff[1][s]] = c2;
       }
+      CM_OMP_FOR
       for (int b = 0; b < B; ++b) {
         RV c[4];
         bend(b, p, c);
         for (int r = 0; r < 4; ++r) slots[tp.boff[r][b]] = c[r];
       }
+      CM_OMP_FOR
       for (int i = 0; i < n; ++i) {
         RV a = mk<R>(0, 0, 0);
         for (int k = tp.prefix[i]; k < tp.prefix[i + 1]; ++k) a = a + slots[k];
@@ -1338,9 +1393,10 @@ struct Sim {
       }
       // dynamic constraints (:1190-1415, :1448-1475)
       if (C) {
-        for (size_t c = 0; c < C; ++c) {
+        CM_OMP_FOR
+        for (long c = 0; c < (long)C; ++c) {
           RV cc[4];
-          if (c < vf.size())
+          if ((size_t)c < vf.size())
             contact_vf(vf[c], p, cc);
           else
             contact_ee(ee[c - vf.size()], p, cc);
@@ -1366,6 +1422,7 @@ struct Sim {
         }
       }
       // FinalCorrectionStep (:1690-1698)
+      CM_OMP_FOR
       for (int i = 0; i < n; ++i) p[i] = p[i] + divs(acc[i], R(cnt[i]));
     }
     if (pk_out) pk_out->push_back(p);

@@ -61,6 +61,18 @@ double orc_time_steps(void* hp, int substeps) {
   return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// OpenMP threads of the following calls on this thread (n < 1: the default);
+// returns the count in effect.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n >= 1) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 int orc_set_sphere(void* hp, const float* c, float r) {
   Handle* h = (Handle*)hp;
   with<float>(h, [&](auto& sim) {

@@ -39,6 +39,8 @@ def lib():
     L.orc_time_steps.argtypes = [vp, i32]
     L.orc_time_steps.restype = C.c_double
     L.orc_set_sphere.argtypes = [vp, vp, C.c_float]
+    L.orc_set_threads.argtypes = [i32]
+    L.orc_set_threads.restype = i32
     L.orc_record.argtypes = [vp, i32]
     L.orc_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.orc_set_state.argtypes = [vp, vp, vp, vp, vp]
@@ -104,6 +106,11 @@ class OracleSim:
 
     def step(self, substeps: int = 1) -> None:
         lib().orc_step(self.h, substeps)
+
+    @staticmethod
+    def set_threads(n: int) -> int:
+        """OpenMP threads of the oracle's loops (results do not depend on it)."""
+        return int(lib().orc_set_threads(int(n)))
 
     def time_steps(self, substeps: int) -> float:
         return lib().orc_time_steps(self.h, substeps)

@@ -191,3 +191,36 @@ def test_topology_counts_grid(oracle_mod):
     assert len(eown) == edges and all(v == 1 for v in eown.values())
     # lowest triangle index wins
     assert tri[0, 3] == 0b111111
+
+
+def test_thread_count_invariance(oracle_mod):
+    """The oracle's OpenMP loops write per index and join in index order: one thread
+    and all threads give bit-identical states, pairs and contacts (a folded cloth
+    with VF / EE contacts, FP32 and FP64)."""
+    n = 20
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    xyz[n // 2:, :, 1] = 0.003
+    xyz[n // 2:, :, 0] = xyz[n - 1 - np.arange(n // 2, n), :, 0]
+    sc["xyz"] = xyz.reshape(-1, 3)
+    out = {}
+    default = oracle_mod.OracleSim.set_threads(0)
+    try:
+        for threads in (1, max(default, 4)):
+            oracle_mod.OracleSim.set_threads(threads)
+            for prec in (32, 64):
+                o = oracle_mod.OracleSim(sc, prec)
+                res = []
+                for _ in range(3):
+                    o.step(1)
+                    x, p, v = o.state()
+                    res.append((x, v, o.pairs(), o.contacts(False), o.contacts(True)))
+                out[(threads, prec)] = res
+    finally:
+        oracle_mod.OracleSim.set_threads(default)
+    for prec in (32, 64):
+        a, b = out[(1, prec)], out[(max(default, 4), prec)]
+        assert sum(len(r[3]) + len(r[4]) for r in a) > 0
+        for ra, rb in zip(a, b):
+            for u, w in zip(ra, rb):
+                assert np.array_equal(u, w)
