@@ -135,10 +135,13 @@ def main():
                                               iters, contacts, counts["pairs"])
     solve_ms = float(avg.value)
     achieved = per_iter / (solve_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, None
     tf = os.path.join(ROOT, "profiles", "k_solve_sphere_drop_256_traffic.json")
     if os.path.exists(tf) and args.size == 256:
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+        tj = json.load(open(tf))
+        traffic = tj.get("bytes_per_launch")
+        traffic_note = (f"rocprofv3 FETCH_SIZE x {tj.get('fetch_scale')} + WRITE_SIZE x "
+                        f"{tj.get('write_scale')} (256 MiB copy calibration), {tj.get('state')}")
     out = dict(
         metric=METRIC,
         value=1e3 / ms_per_step,
@@ -157,7 +160,7 @@ def main():
                     bend_constraints=counts["b"], iterations=iters, substep_dt=1 / 240,
                     morton_bits=30, parallelism="single cloth, one GPU"),
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=achieved / HBM_PEAK_GBS, traffic=traffic,
+                      frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_note,
                       kernel="k_solve (fused Jacobi iteration)", bytes_per_launch=per_iter,
                       avg_launch_ms=solve_ms),
         vertex_updates_per_s=counts["n"] * 1e3 / ms_per_step,

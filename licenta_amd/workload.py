@@ -2,12 +2,17 @@
 
     python -m licenta_amd.workload --size 256 --substeps 640 --kernel-reps 150
 
+    python -m licenta_amd.workload --adjoint --size 128 --substeps 60 --every 10
+
 Creates the 256x256 sphere-drop cloth, writes its initial state back with
 cm_set_state (which synchronises the body's stream before the substep graph is
 captured), runs ``--substeps`` substeps through the captured substep graph, then
 launches the Jacobi kernel ``--kernel-reps`` times on the final state with HIP events
 around it (as bench.py does for its roofline) and prints one JSON line with the
-counts and the average launch time.
+counts and the average launch time. With --adjoint: the fitting cloth (pinned
+corners, stiffness 0.65, zero velocity), then ``--reps`` rounds of a checkpointed
+forward of ``--substeps`` substeps and its adjoint (loss against the ground-truth
+run's final positions).
 """
 from __future__ import annotations
 
@@ -27,7 +32,13 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--substeps", type=int, default=640)
     ap.add_argument("--kernel-reps", type=int, default=150)
+    ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--every", type=int, default=10)
+    ap.add_argument("--reps", type=int, default=2)
     args = ap.parse_args()
+    if args.adjoint:
+        adjoint(args)
+        return
     scene = configs.config_scene(1, args.size)
     body = ClothBody.from_scene(scene)
     body.set_state(scene["xyz"], np.zeros_like(scene["xyz"]))
@@ -40,6 +51,30 @@ def main() -> None:
                solve_avg_ms=float(avg.value), counts=body.counts(), warnings=body.warnings())
     body.close()
     print(json.dumps(out))
+
+
+def adjoint(args) -> None:
+    import time
+
+    sc = configs.config_scene(3, args.size)
+    body = ClothBody.from_scene(sc)
+    tri0 = body.buffer("tri").copy()
+    body.set_state(sc["xyz"], sc["v0"], tri=tri0)
+    body.fixed_update(args.substeps)
+    target = body.positions().astype(np.float64)
+    body.set_stiffness(np.full(body.counts()["s"], 0.65, np.float32))
+    times = []
+    for _ in range(args.reps):
+        body.set_state(sc["xyz"], np.zeros_like(sc["xyz"]), tri=tri0)
+        t0 = time.perf_counter()
+        body.forward_ckpt(args.substeps, args.every)
+        x = body.positions()
+        t1 = time.perf_counter()
+        body.adjoint((2.0 * (x - target) / len(x)).astype(np.float32))
+        times.append((t1 - t0, time.perf_counter() - t1))
+    body.close()
+    print(json.dumps(dict(workload=f"fit_{args.size}x{args.size}", substeps=args.substeps,
+                          every=args.every, forward_and_adjoint_s=times)))
 
 
 if __name__ == "__main__":
