@@ -170,13 +170,18 @@ def main():
         stage_ms_after_run=stage_ms,
     )
     body.close()
+    _progress(f"single-cloth leg: {out['value']:.1f} substeps/s")
     out["batched"] = batched_leg(args.batch_cloths, args.batch_steps) if args.batch_cloths > 0 \
         else None
+    _progress("batched leg done")
     out["large"] = large_leg(args.large_size, args.large_steps) if args.large_size > 0 else None
+    _progress("large-cloth leg done")
     out["adjoint"] = adjoint_leg(args.adj_size, args.adj_substeps, args.adj_every) \
         if args.adj_size > 0 else None
+    _progress("adjoint leg done")
     out["batched_adjoint"] = batched_adjoint_leg(args.adj_batch, 30, 10) \
         if args.adj_batch > 0 else None
+    _progress("batched adjoint leg done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(configs.config_scene(1, args.size), args.cpu_budget)
         out["cpu_baseline"]["cpu_model"] = _cpu_model()
@@ -184,6 +189,10 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def _progress(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:

@@ -91,6 +91,7 @@ def test_adjoint_matches_fp64_oracle(oracle_mod, size, substeps, every):
     ref, _, x64 = _oracle_grads(oracle_mod, sc, body, kd, kb, v0, target, substeps)
     g, x = _gpu_grads(body, target, substeps, every)
     assert np.abs(x - x64).max() <= 1e-5 * np.abs(x64).max()
+    print({k: f"{scaled_err(g[k], ref[k]):.2e}" for k in ("kd", "kb", "x", "v")}, flush=True)
     for k in ("kd", "kb", "x", "v"):
         assert np.abs(ref[k]).max() > 0
         assert scaled_err(g[k], ref[k]) <= TOL, (k, scaled_err(g[k], ref[k]))
@@ -106,6 +107,7 @@ def test_adjoint_with_contacts(oracle_mod):
     assert gpu_counts == counts
     assert sum(a + b for a, b in counts) > 0  # contacts take part
     g, _ = _gpu_grads(body, target, substeps, 2)
+    print({k: f"{scaled_err(g[k], ref[k]):.2e}" for k in ("kd", "kb", "x", "v")}, flush=True)
     for k in ("kd", "kb", "x", "v"):
         assert scaled_err(g[k], ref[k]) <= TOL, (k, scaled_err(g[k], ref[k]))
     body.close()
