@@ -292,6 +292,7 @@ __device__ __forceinline__ void write_node(float4* __restrict__ nodes, int q, in
 
 constexpr int kRefitCH = 1024;  // leaves (and node ids) per refit workgroup
 constexpr int kTopCap = 2048;   // deferred nodes the top pass keeps in LDS
+constexpr int kTopList = 8192;  // pending deferred nodes the top pass lists in LDS
 
 // Refit, pass 1: one workgroup per chunk of 1024 consecutive leaves. Internal node
 // q belongs to the chunk holding index q (a Karras node's index lies in its key
@@ -306,7 +307,7 @@ __global__ void __launch_bounds__(kRefitCH) k_refit_local(
     const int2* __restrict__ ranges, float4* __restrict__ amin, float4* __restrict__ amax,
     int* __restrict__ rmll, int* __restrict__ rmlr, float4* __restrict__ nodes,
     int* __restrict__ state, int* __restrict__ top_list, int* __restrict__ top_index,
-    int* __restrict__ ntop) {
+    int* __restrict__ ntop, int* __restrict__ gdone) {
   __shared__ float4 lmin[kRefitCH], lmax[kRefitCH];  // leaf boxes of the chunk
   __shared__ float4 nmin[kRefitCH], nmax[kRefitCH];  // node boxes of the chunk
   __shared__ int nrm[kRefitCH];                      // max(rmll, rmlr) of each node
@@ -374,6 +375,7 @@ __global__ void __launch_bounds__(kRefitCH) k_refit_local(
     const int j = atomicAdd(ntop, 1);
     top_list[j] = q;
     top_index[q] = j;
+    gdone[j] = 0;  // pending for the later passes
   }
 }
 
@@ -396,6 +398,87 @@ __device__ __forceinline__ void top_child(int c, int nin, const int* __restrict_
   }
 }
 
+// a finished deferred node j (node qq): its box and rightmost leaf in the deferred
+// scratch (for its parent) and in the tree
+__device__ __forceinline__ void put_deferred(int j, int qq, int ll, int rr, int a, int b, f3 lmn,
+                                             f3 lmx, f3 rmn, f3 rmx, float4* __restrict__ gmin,
+                                             float4* __restrict__ gmax, int* __restrict__ grm,
+                                             float4* __restrict__ amin, float4* __restrict__ amax,
+                                             int* __restrict__ rmll, int* __restrict__ rmlr,
+                                             float4* __restrict__ nodes) {
+  gmin[j] = f4(vmin(lmn, rmn), 0.f);
+  gmax[j] = f4(vmax(lmx, rmx), 0.f);
+  grm[j] = max(a, b);
+  amin[qq] = gmin[j];
+  amax[qq] = gmax[j];
+  rmll[qq] = a;
+  rmlr[qq] = b;
+  write_node(nodes, qq, ll, rr, a, b, lmn, lmx, rmn, rmx);
+}
+
+constexpr int kSuper = 64 * kRefitCH;  // leaves per workgroup of the middle pass
+constexpr int kMidCap = 8192;          // deferred nodes one middle workgroup holds
+
+// Refit, middle pass (large trees: more deferred nodes than the top pass holds in
+// LDS): one workgroup per range of kSuper leaves finishes the deferred nodes whose
+// key range lies inside it, with the synchronised passes of k_refit_local over the
+// deferred scratch. A deferred child outside the range (possible only in a broken
+// tree) leaves its parent to the top pass. Finished nodes are marked gdone = -pass;
+// the top pass treats them as finished before its first pass.
+__global__ void __launch_bounds__(1024) k_refit_mid(
+    int t, const int* __restrict__ lefts, const int* __restrict__ rights,
+    const int2* __restrict__ ranges, float4* __restrict__ amin, float4* __restrict__ amax,
+    int* __restrict__ rmll, int* __restrict__ rmlr, float4* __restrict__ nodes,
+    const int* __restrict__ state, const int* __restrict__ top_list,
+    const int* __restrict__ top_index, const int* __restrict__ ntop, float4* __restrict__ gmin,
+    float4* __restrict__ gmax, int* __restrict__ grm, int* __restrict__ gdone) {
+  __shared__ int own[kMidCap];
+  __shared__ int nown;
+  const int nin = t - 1, tid = threadIdx.x;
+  const int nt = *ntop;
+  if (nt <= kTopCap) return;  // the top pass keeps a short list in LDS
+  const int sbase = blockIdx.x * kSuper, send = sbase + kSuper;
+  if (tid == 0) nown = 0;
+  __syncthreads();
+  for (int j = tid; j < nt; j += 1024) {
+    const int2 rg = ranges[top_list[j]];
+    if (rg.x >= sbase && rg.y < send) {
+      const int k = atomicAdd(&nown, 1);
+      if (k < kMidCap) own[k] = j;
+    }
+  }
+  __syncthreads();
+  const int no = nown;
+  if (no > kMidCap) return;  // too many: all of them are left to the top pass
+  auto inside = [&](int c) {
+    const int2 rg = ranges[c];
+    return rg.x >= sbase && rg.y < send;
+  };
+  for (int pass = 1;; ++pass) {
+    bool changed = false;
+    for (int k = tid; k < no; k += 1024) {
+      const int j = own[k];
+      if (gdone[j] != 0) continue;
+      const int qq = top_list[j], ll = lefts[qq], rr = rights[qq];
+      bool lt, rt;
+      int a, b;
+      f3 lmn, lmx, rmn, rmx;
+      top_child(ll, nin, state, top_index, amin, amax, rmll, rmlr, lt, a, lmn, lmx);
+      top_child(rr, nin, state, top_index, amin, amax, rmll, rmlr, rt, b, rmn, rmx);
+      if (lt && !(inside(ll) && gdone[a] < 0 && -gdone[a] < pass)) continue;
+      if (rt && !(inside(rr) && gdone[b] < 0 && -gdone[b] < pass)) continue;
+      if (lt) { lmn = xyz(gmin[a]); lmx = xyz(gmax[a]); a = grm[a]; }
+      if (rt) { rmn = xyz(gmin[b]); rmx = xyz(gmax[b]); b = grm[b]; }
+      put_deferred(j, qq, ll, rr, a, b, lmn, lmx, rmn, rmx, gmin, gmax, grm, amin, amax, rmll,
+                   rmlr, nodes);
+      __threadfence_block();
+      gdone[j] = -pass;
+      changed = true;
+    }
+    if (!__syncthreads_or(changed)) break;
+  }
+}
+
 // Refit, pass 2: one workgroup finishes the deferred nodes with the same
 // synchronised passes. Up to kTopCap deferred nodes keep their boxes in LDS and
 // their children in registers (two per thread); beyond that the boxes live in
@@ -409,6 +492,8 @@ __global__ void __launch_bounds__(1024) k_refit_top(
     int* __restrict__ gdone, uint32_t* __restrict__ warn) {
   __shared__ float4 smin[kTopCap], smax[kTopCap];
   __shared__ int srm[kTopCap], sdone[kTopCap];
+  __shared__ int plist[kTopList];
+  __shared__ int npend;
   const int nin = t - 1, tid = threadIdx.x;
   const int nt = *ntop;
   bool left_over = false;
@@ -473,11 +558,24 @@ __global__ void __launch_bounds__(1024) k_refit_top(
       write_node(nodes, q[k], l[k], r[k], la[k], ra[k], lmn[k], lmx[k], rmn[k], rmx[k]);
     }
   } else {
-    for (int j = tid; j < nt; j += 1024) gdone[j] = 0;
+    // gdone: 0 pending (set by k_refit_local), < 0 finished by k_refit_mid, > 0 the
+    // pass of this kernel that finished it. The pending nodes are listed in LDS when
+    // they fit, so the passes skip the finished ones.
+    if (tid == 0) npend = 0;
     __syncthreads();
+    for (int j = tid; j < nt; j += 1024)
+      if (gdone[j] == 0) {
+        const int k = atomicAdd(&npend, 1);
+        if (k < kTopList) plist[k] = j;
+      }
+    __syncthreads();
+    const int np = npend;
+    const bool listed = np <= kTopList;
+    const int nloop = listed ? np : nt;
     for (int pass = 1;; ++pass) {
       bool changed = false;
-      for (int j = tid; j < nt; j += 1024) {
+      for (int k = tid; k < nloop; k += 1024) {
+        const int j = listed ? plist[k] : k;
         if (gdone[j] != 0) continue;
         const int qq = top_list[j], ll = lefts[qq], rr = rights[qq];
         bool lt, rt;
@@ -489,22 +587,16 @@ __global__ void __launch_bounds__(1024) k_refit_top(
         if (rt && !(gdone[b] != 0 && gdone[b] < pass)) continue;
         if (lt) { lmn = xyz(gmin[a]); lmx = xyz(gmax[a]); a = grm[a]; }
         if (rt) { rmn = xyz(gmin[b]); rmx = xyz(gmax[b]); b = grm[b]; }
-        gmin[j] = f4(vmin(lmn, rmn), 0.f);
-        gmax[j] = f4(vmax(lmx, rmx), 0.f);
-        grm[j] = max(a, b);
-        amin[qq] = gmin[j];
-        amax[qq] = gmax[j];
-        rmll[qq] = a;
-        rmlr[qq] = b;
-        write_node(nodes, qq, ll, rr, a, b, lmn, lmx, rmn, rmx);
+        put_deferred(j, qq, ll, rr, a, b, lmn, lmx, rmn, rmx, gmin, gmax, grm, amin, amax, rmll,
+                     rmlr, nodes);
         __threadfence_block();
         gdone[j] = pass;
         changed = true;
       }
       if (!__syncthreads_or(changed)) break;
     }
-    for (int j = tid; j < nt; j += 1024)
-      if (gdone[j] == 0) left_over = true;
+    for (int k = tid; k < nloop; k += 1024)
+      if (gdone[listed ? plist[k] : k] == 0) left_over = true;
   }
   if (__syncthreads_or(left_over) && tid == 0) atomicOr(warn, (uint32_t)CM_WARN_REFIT_LIMIT);
   if (tid == 0) *ntop = 0;
@@ -641,22 +733,48 @@ __global__ void __launch_bounds__(kBlock) k_pack_nodes(int t, const int* __restr
 // One wave per leaf copies its chunk to the compacted list. The leaf's offset is its
 // tile's prefix plus the counts of the tile's earlier leaves (at most 127, summed by
 // the wave).
+// One wave per traversal tile of kTravBlock leaves: the leaves' counts are scanned
+// across the wave, then the wave writes the tile's pairs in order, 64 consecutive
+// output slots at a time (each lane finds its slot's leaf by binary search over the
+// tile's prefixes), so reads and writes are contiguous runs.
+constexpr int kCompactWaves = kBlock / 64;
 __global__ void __launch_bounds__(kBlock) k_compact_pairs(int t, int cap,
                                                           const int2* __restrict__ chunk,
                                                           const int* __restrict__ counts,
                                                           const int* __restrict__ tile_pre,
                                                           int2* __restrict__ pairs) {
-  const int leaf = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (leaf >= t) return;
-  const int first = leaf / kTravBlock * kTravBlock;
-  int before = 0;
-  for (int j = first + lane; j < leaf; j += 64) before += counts[j];
+  static_assert(kTravBlock == 128, "two leaves per lane");
+  __shared__ int pre[kCompactWaves][kTravBlock + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ntile = (t + kTravBlock - 1) / kTravBlock;
+  const int tile = blockIdx.x * kCompactWaves + w;
+  const bool live = tile < ntile;
+  const int l0 = tile * kTravBlock;
+  const int i0 = l0 + 2 * lane, i1 = i0 + 1;
+  const int c0 = live && i0 < t ? counts[i0] : 0, c1 = live && i1 < t ? counts[i1] : 0;
+  int s = c0 + c1;
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) before += __shfl_xor(before, o, 64);
-  const int c = counts[leaf], o = tile_pre[leaf / kTravBlock] + before;
-  const int2* src = chunk + (int64_t)leaf * cap;
-  for (int k = lane; k < c; k += 64) pairs[o + k] = src[k];
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(s, o, 64);
+    if (lane >= o) s += y;
+  }
+  const int ex = s - (c0 + c1);
+  pre[w][2 * lane] = ex;
+  pre[w][2 * lane + 1] = ex + c0;
+  if (lane == 63) pre[w][kTravBlock] = s;
+  __syncthreads();
+  if (!live) return;
+  const int total = pre[w][kTravBlock];
+  const int out0 = tile_pre[tile];
+  for (int k = lane; k < total; k += 64) {
+    int lo = 0, hi = kTravBlock - 1;  // the last leaf whose prefix is <= k holds slot k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[w][mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    pairs[out0 + k] = chunk[(int64_t)(l0 + lo) * cap + (k - pre[w][lo])];
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_gather_tris(int t, const uint32_t* __restrict__ perm,
@@ -746,7 +864,11 @@ int launch_refit(const LbvhBufs& b, hipStream_t s) {
   if (t < 2) return CM_OK;
   hipLaunchKernelGGL(k_refit_local, dim3((t + kRefitCH - 1) / kRefitCH), dim3(kRefitCH), 0, s, t,
                      b.lefts, b.rights, b.ranges, b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.state,
-                     b.top_list, b.top_index, b.ntop);
+                     b.top_list, b.top_index, b.ntop, b.tdone);
+  if (t > 2 * kSuper)  // smaller trees defer at most a few hundred nodes
+    hipLaunchKernelGGL(k_refit_mid, dim3((t + kSuper - 1) / kSuper), dim3(1024), 0, s, t, b.lefts,
+                       b.rights, b.ranges, b.amin, b.amax, b.rmll, b.rmlr, b.nodes, b.state,
+                       b.top_list, b.top_index, b.ntop, b.tmin, b.tmax, b.trm, b.tdone);
   hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, t, b.lefts, b.rights, b.amin, b.amax,
                      b.rmll, b.rmlr, b.nodes, b.state, b.top_list, b.top_index, b.ntop, b.tmin,
                      b.tmax, b.trm, b.tdone, b.warn);
@@ -768,8 +890,8 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   int rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
                       nullptr, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_compact_pairs, dim3(nblocks((int64_t)t * 64)), dim3(kBlock), 0, s, t, cap,
-                     chunk, counts, offsets + nt, pairs);
+  hipLaunchKernelGGL(k_compact_pairs, dim3((nt + kCompactWaves - 1) / kCompactWaves), dim3(kBlock),
+                     0, s, t, cap, chunk, counts, offsets + nt, pairs);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

@@ -3,6 +3,7 @@
     python -m licenta_amd.workload --size 256 --substeps 640 --kernel-reps 150
 
     python -m licenta_amd.workload --adjoint --size 128 --substeps 60 --every 10
+    python -m licenta_amd.workload --batch 512 --substeps 8
 
 Creates the 256x256 sphere-drop cloth, writes its initial state back with
 cm_set_state (which synchronises the body's stream before the substep graph is
@@ -35,9 +36,14 @@ def main() -> None:
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--every", type=int, default=10)
     ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=0,
+                    help="step the N-cloth batch of uneven cloths instead (seed 2)")
     args = ap.parse_args()
     if args.adjoint:
         adjoint(args)
+        return
+    if args.batch:
+        batch(args)
         return
     scene = configs.config_scene(1, args.size)
     body = ClothBody.from_scene(scene)
@@ -51,6 +57,18 @@ def main() -> None:
                solve_avg_ms=float(avg.value), counts=body.counts(), warnings=body.warnings())
     body.close()
     print(json.dumps(out))
+
+
+def batch(args) -> None:
+    scenes = configs.batched_scene(2, count=args.batch)
+    body = ClothBody.batch(scenes)
+    x0 = np.concatenate([s["xyz"] for s in scenes])
+    body.set_state(x0, np.zeros_like(x0))
+    body.fixed_update(args.substeps)
+    body.sync()
+    print(json.dumps(dict(workload=f"batch of {args.batch} cloths", substeps=args.substeps,
+                          counts=body.counts(), warnings=body.warnings())))
+    body.close()
 
 
 def adjoint(args) -> None:

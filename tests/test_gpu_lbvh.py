@@ -76,10 +76,15 @@ def make_tree_inputs(t, seed, ties):
     return keys, lo, hi
 
 
+# 300,000 leaves: more than 2,048 deferred nodes, so the refit's middle pass runs
+# (with tied codes the midpoint split makes a broken tree, whose shared nodes can
+# leave parents to the top pass)
 @pytest.mark.parametrize("t,ties,midpoint", [(2, False, True), (777, False, True),
                                              (5000, True, True), (5000, True, False),
-                                             (40000, False, True)])
+                                             (40000, False, True), (300000, False, True),
+                                             (300000, True, True)])
 def test_build_refit_pairs(oracle_mod, t, ties, midpoint):
+    print(f"build/refit/pairs t={t} ties={ties} midpoint={midpoint}", flush=True)
     keys, lo, hi = make_tree_inputs(t, t, ties)
     L = lib()
     s = stream()
