@@ -733,47 +733,44 @@ __global__ void __launch_bounds__(kBlock) k_pack_nodes(int t, const int* __restr
 // One wave per leaf copies its chunk to the compacted list. The leaf's offset is its
 // tile's prefix plus the counts of the tile's earlier leaves (at most 127, summed by
 // the wave).
-// One wave per traversal tile of kTravBlock leaves: the leaves' counts are scanned
-// across the wave, then the wave writes the tile's pairs in order, 64 consecutive
-// output slots at a time (each lane finds its slot's leaf by binary search over the
-// tile's prefixes), so reads and writes are contiguous runs.
-constexpr int kCompactWaves = kBlock / 64;
+// One workgroup per traversal tile of kTravBlock leaves: the first wave scans the
+// leaves' counts, then the workgroup writes the tile's pairs in order, kBlock
+// consecutive output slots at a time (each thread finds its slot's leaf by binary
+// search over the tile's prefixes), so reads and writes are contiguous runs.
 __global__ void __launch_bounds__(kBlock) k_compact_pairs(int t, int cap,
                                                           const int2* __restrict__ chunk,
                                                           const int* __restrict__ counts,
                                                           const int* __restrict__ tile_pre,
                                                           int2* __restrict__ pairs) {
-  static_assert(kTravBlock == 128, "two leaves per lane");
-  __shared__ int pre[kCompactWaves][kTravBlock + 1];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ntile = (t + kTravBlock - 1) / kTravBlock;
-  const int tile = blockIdx.x * kCompactWaves + w;
-  const bool live = tile < ntile;
+  static_assert(kTravBlock == 128, "two leaves per lane of the first wave");
+  __shared__ int pre[kTravBlock + 1];
+  const int tile = blockIdx.x, tid = threadIdx.x;
   const int l0 = tile * kTravBlock;
-  const int i0 = l0 + 2 * lane, i1 = i0 + 1;
-  const int c0 = live && i0 < t ? counts[i0] : 0, c1 = live && i1 < t ? counts[i1] : 0;
-  int s = c0 + c1;
+  if (tid < 64) {
+    const int i0 = l0 + 2 * tid, i1 = i0 + 1;
+    const int c0 = i0 < t ? counts[i0] : 0, c1 = i1 < t ? counts[i1] : 0;
+    int s = c0 + c1;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(s, o, 64);
-    if (lane >= o) s += y;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(s, o, 64);
+      if (tid >= o) s += y;
+    }
+    const int ex = s - (c0 + c1);
+    pre[2 * tid] = ex;
+    pre[2 * tid + 1] = ex + c0;
+    if (tid == 63) pre[kTravBlock] = s;
   }
-  const int ex = s - (c0 + c1);
-  pre[w][2 * lane] = ex;
-  pre[w][2 * lane + 1] = ex + c0;
-  if (lane == 63) pre[w][kTravBlock] = s;
   __syncthreads();
-  if (!live) return;
-  const int total = pre[w][kTravBlock];
+  const int total = pre[kTravBlock];
   const int out0 = tile_pre[tile];
-  for (int k = lane; k < total; k += 64) {
+  for (int k = tid; k < total; k += kBlock) {
     int lo = 0, hi = kTravBlock - 1;  // the last leaf whose prefix is <= k holds slot k
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (pre[w][mid] <= k) lo = mid;
+      if (pre[mid] <= k) lo = mid;
       else hi = mid - 1;
     }
-    pairs[out0 + k] = chunk[(int64_t)(l0 + lo) * cap + (k - pre[w][lo])];
+    pairs[out0 + k] = chunk[(int64_t)(l0 + lo) * cap + (k - pre[lo])];
   }
 }
 
@@ -890,8 +887,8 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   int rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
                       nullptr, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_compact_pairs, dim3((nt + kCompactWaves - 1) / kCompactWaves), dim3(kBlock),
-                     0, s, t, cap, chunk, counts, offsets + nt, pairs);
+  hipLaunchKernelGGL(k_compact_pairs, dim3(nt), dim3(kBlock), 0, s, t, cap, chunk, counts,
+                     offsets + nt, pairs);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
