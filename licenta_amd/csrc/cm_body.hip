@@ -804,6 +804,7 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
       (rc = dalloc(b, &nb.dlong, (size_t)n)))
     return fail(rc);
   CM_HIP(hipMemset(nb.dprefix, 0, sizeof(int) * ((size_t)n + 1)));
+  CM_HIP(hipMemset(nb.dcount, 0, sizeof(int) * ((size_t)n + 1)));  // kept zero between substeps
   if (params->flags & CM_F_SPHERE) {
     b->prm.flags |= CM_F_SPHERE;
   }

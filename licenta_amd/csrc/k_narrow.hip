@@ -766,8 +766,9 @@ __global__ void __launch_bounds__(kBlock) k_dyn_count(const int4* __restrict__ v
                                                       const int4* __restrict__ eev,
                                                       const int* __restrict__ d_nvf,
                                                       const int* __restrict__ d_nee,
-                                                      int* __restrict__ dcount) {
+                                                      int* __restrict__ dcount, int n) {
   const int nvf = *d_nvf, nee = *d_nee;
+  if (blockIdx.x == 0 && threadIdx.x == 0) dcount[n] = 0;  // long-list counter of k_dyn_sort
   for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
     const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
     atomicAdd(&dcount[q.x], 1);
@@ -781,13 +782,17 @@ __global__ void __launch_bounds__(kBlock) k_dyn_fill(const int4* __restrict__ vf
                                                      const int4* __restrict__ eev,
                                                      const int* __restrict__ d_nvf,
                                                      const int* __restrict__ d_nee,
-                                                     int* __restrict__ cursor,
+                                                     const int* __restrict__ prefix,
+                                                     int* __restrict__ dcount,
                                                      uint32_t* __restrict__ drec) {
+  // each vertex's count runs down to zero as its records are placed (in any order:
+  // k_dyn_sort orders them), which leaves the counts cleared for the next substep
   const int nvf = *d_nvf, nee = *d_nee;
   for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
     const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
     const int vv[4] = {q.x, q.y, q.z, q.w};
-    for (int r = 0; r < 4; ++r) drec[atomicAdd(&cursor[vv[r]], 1)] = 4u * (uint32_t)c + r;
+    for (int r = 0; r < 4; ++r)
+      drec[prefix[vv[r]] + atomicSub(&dcount[vv[r]], 1) - 1] = 4u * (uint32_t)c + r;
   }
 }
 
@@ -958,18 +963,16 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
 }
 
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
-  int rc = fill_i32(nb.dcount, 0, (int64_t)nb.n + 1, s);  // dcount[n]: long-list counter
-  if (rc) return rc;
+  // dcount[0, n) is zero here: cleared at creation, then by every k_dyn_fill
   const int gc = grid_for(nb.vf_out_cap + nb.ee_out_cap);
   hipLaunchKernelGGL(k_dyn_count, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
-                     nb.d_nee, nb.dcount);
+                     nb.d_nee, nb.dcount, nb.n);
   CM_LAUNCH_CHECK();
-  rc = scan_exclusive(nb.dcount, nb.dprefix, nb.n, nullptr, nb.dprefix + nb.n, nb.scan_temp, s);
+  int rc = scan_exclusive(nb.dcount, nb.dprefix, nb.n, nullptr, nb.dprefix + nb.n, nb.scan_temp,
+                          s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_copy_i32, dim3(grid_for(nb.n)), dim3(kBlock), 0, s, nb.dprefix, nb.dcursor,
-                     nb.n);
   hipLaunchKernelGGL(k_dyn_fill, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
-                     nb.d_nee, nb.dcursor, nb.drec);
+                     nb.d_nee, nb.dprefix, nb.dcount, nb.drec);
   hipLaunchKernelGGL(k_dyn_sort, dim3(nblocks(nb.n)), dim3(kBlock), 0, s, nb.n, nb.dprefix,
                      nb.drec, nb.dcount + nb.n, nb.dlong, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v,
                      nb.ee_w, nb.rec);

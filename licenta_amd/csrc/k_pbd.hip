@@ -1093,8 +1093,9 @@ int launch_solver(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, h
   const int g = nblocks(b.n);
   const int ground_on = (pp.flags & CM_F_GROUND) ? 1 : 0;
   const int F = (pp.flags & CM_F_LRA) ? b.F : 0;
-  hipLaunchKernelGGL(k_setup, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.gh, pp.th, ground_on,
-                     b.ground, F, b.lra_fixed, b.lra_l0, b.pinned, b.lra_flag);
+  if (ground_on || F)  // otherwise the set-up writes nothing
+    hipLaunchKernelGGL(k_setup, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.gh, pp.th,
+                       ground_on, b.ground, F, b.lra_fixed, b.lra_l0, b.pinned, b.lra_flag);
   if (pp.flags & CM_F_SPHERE)
     hipLaunchKernelGGL(k_sphere, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.sc, pp.sr + pp.th);
   CM_LAUNCH_CHECK();
@@ -1124,8 +1125,9 @@ int launch_solver_rec(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& n
   const int g = nblocks(b.n);
   const int ground_on = (pp.flags & CM_F_GROUND) ? 1 : 0;
   const int F = (pp.flags & CM_F_LRA) ? b.F : 0;
-  hipLaunchKernelGGL(k_setup, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.gh, pp.th, ground_on,
-                     b.ground, F, b.lra_fixed, b.lra_l0, b.pinned, b.lra_flag);
+  if (ground_on || F)  // otherwise the set-up writes nothing
+    hipLaunchKernelGGL(k_setup, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.gh, pp.th,
+                       ground_on, b.ground, F, b.lra_fixed, b.lra_l0, b.pinned, b.lra_flag);
   if (pp.flags & CM_F_SPHERE)
     hipLaunchKernelGGL(k_sphere, dim3(g), dim3(kBlock), 0, s, b.n, b.x, b.p, pp.sc, pp.sr + pp.th);
   CM_LAUNCH_CHECK();
