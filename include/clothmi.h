@@ -255,6 +255,16 @@ int cm_forward_ckpt(cm_body* body, int32_t substeps, int32_t every);
 int cm_adjoint(cm_body* body, const float* dl_dx, const float* dl_dv, float* dl_dx0,
                float* dl_dv0, float* dl_dk_dist, float* dl_dk_bend);
 
+/* ---- launch-shape autotuner ------------------------------------------------
+ * No counterpart in the reference (its launch shapes are fixed). Times `substeps`
+ * substeps from the current state for each candidate of, one at a time: the Jacobi
+ * tile size (64, 128 or 256 vertices), the traversal's tiles per XCD chunk (1, 2, 4,
+ * 8) and the secant pass's workgroups (1024, 2048, 4096), keeps the fastest of each,
+ * and restores the state (the simulation does not advance). None of them changes a
+ * result. chosen (3 ints, may be NULL) receives the picks in that order. Syncs; the
+ * substep graph is captured again at the next cm_step. */
+int cm_autotune(cm_body* body, int32_t substeps, int32_t* chosen);
+
 /* ---- unit-level LBVH entry points on caller device buffers -------------- */
 /* Swept leaf AABBs (written to leaf slots [T-1, 2T-1) of d_aabb_min/max) and
  * Morton keys for triangles d_tri in their current order. */

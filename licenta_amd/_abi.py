@@ -122,6 +122,7 @@ def lib():
     L.cm_set_stiffness.argtypes = [vp, vp, vp]
     L.cm_forward_ckpt.argtypes = [vp, i32, i32]
     L.cm_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.cm_autotune.argtypes = [vp, i32, vp]
     _lib = L
     return L
 
@@ -141,5 +142,5 @@ EXPORTS = (
     "cm_query_contacts", "cm_tri_update", "cm_lbvh_sort_temp_bytes", "cm_lbvh_sort",
     "cm_lbvh_build_temp_bytes", "cm_lbvh_build", "cm_lbvh_refit", "cm_lbvh_pairs_temp_bytes",
     "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times", "cm_set_stiffness", "cm_forward_ckpt",
-    "cm_adjoint",
+    "cm_adjoint", "cm_autotune",
 )

@@ -179,6 +179,14 @@ class ClothBody:
     def pairs(self) -> np.ndarray:
         return self.buffer("pairs")
 
+    def autotune(self, substeps: int = 3) -> dict:
+        """Pick the fastest launch shapes on the current state (cm_autotune); the
+        state is restored and no result changes."""
+        out = np.zeros(3, dtype=np.int32)
+        check(lib().cm_autotune(self.h, int(substeps), ptr(out)))
+        return dict(tile_vertices=int(out[0]), traversal_chunk=int(out[1]),
+                    secant_workgroups=int(out[2]))
+
     # ---- differentiable rollout ------------------------------------------------------
     def set_stiffness(self, k_dist=None, k_bend=None) -> None:
         """Per-constraint distance / bend stiffness in [0, 1] (None: kept)."""

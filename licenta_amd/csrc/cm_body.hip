@@ -115,6 +115,9 @@ struct cm_body {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
+  // launch shapes (cm_autotune): Jacobi tile vertices; traversal tiles per XCD chunk
+  // and the secant pass's workgroups live in lb / nb
+  int tile_v = kTileV;
   // host copies
   std::vector<float> inv_mass_h;
   std::vector<uint8_t> pinned_h;
@@ -202,12 +205,12 @@ inline uint64_t edge_key(int a, int b) {
 // slots, and list for each tile every constraint that touches it once, with the
 // tile-local slot of each of its roles. Leaves the tiles empty when one vertex has
 // more slots than a tile holds.
-void build_tiles(int n, HostTopo& h) {
+void build_tiles(int n, HostTopo& h, int tile_v = kTileV) {
   const std::vector<int>& pre = h.prefix;
   std::vector<int> ts{0};
   for (int v = 0; v < n;) {
     const int start = v;
-    while (v < n && v - start < kTileV && pre[v + 1] - pre[start] <= kTileSlots) ++v;
+    while (v < n && v - start < tile_v && pre[v + 1] - pre[start] <= kTileSlots) ++v;
     if (v == start) return;  // a vertex with more than kTileSlots slots
     ts.push_back(v);
   }
@@ -1349,7 +1352,7 @@ int cm_set_stiffness(cm_body* b, const float* k_dist, const float* k_bend) {
     h.k_bend[i] = k_bend[i];
     h.b_kpphi0[i].x = kp_of(k_bend[i], I);
   }
-  if (b->ntiles > 0) build_tiles(b->n, h);
+  if (b->ntiles > 0) build_tiles(b->n, h, b->tile_v);
   CM_HIP(hipStreamSynchronize(b->stream));
   if (b->S)
     CM_HIP(hipMemcpy(b->s_l0kp, h.s_l0kp.data(), sizeof(float2) * b->S, hipMemcpyHostToDevice));
@@ -1465,6 +1468,168 @@ int cm_adjoint(cm_body* b, const float* dl_dx, const float* dl_dv, float* dl_dx0
     dl_dk_dist[i] = (float)(gs[i] * dkp_dk(b->topo.k_dist[i], I));
   for (int i = 0; dl_dk_bend && i < b->B; ++i)
     dl_dk_bend[i] = (float)(gb[i] * dkp_dk(b->topo.k_bend[i], I));
+  return CM_OK;
+}
+
+}  // extern "C"
+
+// ---- launch-shape autotuner (no reference counterpart) ---------------------------
+namespace {
+
+// frees a buffer of the body's allocation list
+void free_alloc(cm_body* b, void* ptr) {
+  for (size_t k = 0; k < b->allocs.size(); ++k)
+    if (b->allocs[k] == ptr) {
+      hipFree(ptr);
+      b->allocs.erase(b->allocs.begin() + k);
+      return;
+    }
+}
+
+void drop_graph(cm_body* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Jacobi tiles of tile_v vertices: rebuilt on the host and uploaded in place of the
+// current ones (the substep graph is dropped: it holds the old buffers)
+int set_tiles(cm_body* b, int tile_v) {
+  HostTopo& h = b->topo;
+  build_tiles(b->n, h, tile_v);
+  if (h.tstart.empty()) {
+    set_error("cm_autotune: the mesh has no tiled Jacobi layout");
+    return CM_ERR_STATE;
+  }
+  CM_HIP(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  free_alloc(b, b->tstart);
+  free_alloc(b, b->eprefix);
+  free_alloc(b, b->entries);
+  free_alloc(b, b->adj.ecid);
+  int rc;
+  if ((rc = upload(b, &b->tstart, h.tstart)) || (rc = upload(b, &b->eprefix, h.eprefix)) ||
+      (rc = upload(b, &b->entries, h.entries)) || (rc = upload(b, &b->adj.ecid, h.ecid)))
+    return rc;
+  b->adj.ab.ecid = b->adj.ecid;
+  b->adj.ready = false;
+  b->ntiles = (int)h.tstart.size() - 1;
+  b->tile_v = tile_v;
+  return CM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (substeps < 1) {
+    set_error("cm_autotune: substeps must be positive");
+    return CM_ERR_ARG;
+  }
+  const size_t n = (size_t)b->n, t = (size_t)b->t;
+  CM_HIP(hipStreamSynchronize(b->stream));
+  drop_graph(b);
+  // every trial starts from the current state
+  float4 *sx = nullptr, *sv = nullptr, *sp = nullptr;
+  int4* stri = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&]() {
+    hipFree(sx);
+    hipFree(sv);
+    hipFree(sp);
+    hipFree(stri);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+  };
+  if (hipMalloc(&sx, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&sv, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&sp, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&stri, sizeof(int4) * t) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    set_error("cm_autotune: allocation failed");
+    return CM_ERR_ALLOC;
+  }
+  auto copy_state = [&](bool save) -> hipError_t {
+    hipError_t e = hipSuccess;
+    auto cp = [&](void* keep, void* live, size_t bytes) {
+      if (e == hipSuccess)
+        e = save ? hipMemcpyAsync(keep, live, bytes, hipMemcpyDeviceToDevice, b->stream)
+                 : hipMemcpyAsync(live, keep, bytes, hipMemcpyDeviceToDevice, b->stream);
+    };
+    cp(sx, b->x, sizeof(float4) * n);
+    cp(sv, b->v, sizeof(float4) * n);
+    cp(sp, b->p, sizeof(float4) * n);
+    cp(stri, b->tri, sizeof(int4) * t);
+    return e;
+  };
+  auto trial = [&](float* ms) -> int {
+    if (copy_state(false) != hipSuccess) {
+      set_error("cm_autotune: state copy failed");
+      return CM_ERR_HIP;
+    }
+    CM_HIP(hipEventRecord(e0, b->stream));
+    for (int s = 0; s < substeps; ++s) {
+      const int r = record_substep(b);
+      if (r) return r;
+    }
+    CM_HIP(hipEventRecord(e1, b->stream));
+    CM_HIP(hipEventSynchronize(e1));
+    CM_HIP(hipEventElapsedTime(ms, e0, e1));
+    return CM_OK;
+  };
+  auto fail = [&](int code) {
+    cleanup();
+    return code;
+  };
+  if (copy_state(true) != hipSuccess) return fail(CM_ERR_HIP);
+  float ms = 0.f;
+  if ((rc = trial(&ms))) return fail(rc);  // warm-up
+  // one parameter at a time, each over its candidates, keeping the fastest
+  if (b->ntiles > 0) {
+    const int cand[3] = {64, 128, 256};
+    int best = b->tile_v;
+    float best_ms = 1e30f;
+    for (int c : cand) {
+      if ((rc = set_tiles(b, c)) || (rc = trial(&ms))) return fail(rc);
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    if ((rc = set_tiles(b, best))) return fail(rc);
+  }
+  {
+    const int cand[4] = {1, 2, 4, 8};
+    int best = b->lb.trav_chunk;
+    float best_ms = 1e30f;
+    for (int c : cand) {
+      b->lb.trav_chunk = c;
+      if ((rc = trial(&ms))) return fail(rc);
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    b->lb.trav_chunk = best;
+  }
+  {
+    const int cand[3] = {1024, 2048, 4096};
+    int best = b->nb.secant_grid;
+    float best_ms = 1e30f;
+    for (int c : cand) {
+      b->nb.secant_grid = c;
+      if ((rc = trial(&ms))) return fail(rc);
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    b->nb.secant_grid = best;
+  }
+  if (copy_state(false) != hipSuccess) return fail(CM_ERR_HIP);
+  CM_HIP(hipStreamSynchronize(b->stream));
+  cleanup();
+  if (chosen) {
+    chosen[0] = b->tile_v;
+    chosen[1] = b->lb.trav_chunk;
+    chosen[2] = b->nb.secant_grid;
+  }
   return CM_OK;
 }
 

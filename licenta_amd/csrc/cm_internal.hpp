@@ -82,6 +82,7 @@ struct LbvhBufs {
   float4 *tmin = nullptr, *tmax = nullptr;  // deferred-node scratch (large trees)
   int *trm = nullptr, *tdone = nullptr;
   float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
+  int trav_chunk = 2;       // traversal: tiles per chunk dealt to one XCD (launch shape only)
   uint32_t* warn = nullptr;
   // batch of cloths (ncloth == 0: one cloth): cloth c owns triangle slots (and, after
   // the sort, leaves) [tstart[c], tstart[c + 1]); its Morton keys carry c above bit 30
@@ -155,6 +156,7 @@ struct NarrowBufs {
   int* d_nrec = nullptr;
   uint32_t* warn = nullptr;
   void* scan_temp = nullptr;
+  int secant_grid = 2048;  // workgroups of the secant pass (launch shape only)
 };
 int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
                   const float4* v, float th, float dt, int ee_on, hipStream_t s);

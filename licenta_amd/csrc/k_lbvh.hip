@@ -605,7 +605,6 @@ __global__ void __launch_bounds__(1024) k_refit_top(
 // ---- traversal ----------------------------------------------------------------
 constexpr int kTravBlock = 128;  // 2 waves; the 63-entry stacks take 31.5 KiB of LDS
 constexpr int kStack = 63;       // usable entries, as the reference's int[64] with its -1 sentinel
-constexpr int kTravChunk = 2;    // tiles per chunk of the traversal's XCD mapping
 
 __device__ __forceinline__ void put_pair(int2* p, int a, int b) {
   // streamed past the caches so the node records stay resident in L2
@@ -689,16 +688,17 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int t, const float4* __
                                                          const int* __restrict__ slot_cloth,
                                                          const int* __restrict__ croot,
                                                          const int* __restrict__ ccap,
-                                                         int* __restrict__ tile_sum) {
+                                                         int* __restrict__ tile_sum,
+                                                         int xchunk) {
   // Entry k of every lane's stack is one row, so a wave's pushes and pops at equal
   // depths hit 64 consecutive banks.
   __shared__ int sstk[kStack][kTravBlock];
   // Workgroups are dealt to the 8 XCDs in turn. Each XCD takes every eighth chunk of
-  // kTravChunk consecutive tiles (Morton-ordered leaves), so its L2 serves compact
+  // `xchunk` consecutive tiles (Morton-ordered leaves), so its L2 serves compact
   // subtrees while the dense regions of the cloth, whose leaves visit many more
   // nodes, are shared evenly between the XCDs. The grid is padded to whole rounds.
   const int j = blockIdx.x / kXcds, xcd = blockIdx.x % kXcds;
-  const int tile = ((j / kTravChunk) * kXcds + xcd) * kTravChunk + j % kTravChunk;
+  const int tile = ((j / xchunk) * kXcds + xcd) * xchunk + j % xchunk;
   if (tile >= (t + kTravBlock - 1) / kTravBlock) return;
   const int leaf = tile * kTravBlock + threadIdx.x;
   const int ncol = leaf < t ? leaf_pairs(t, leaf, nodes, amin, amax, cap, chunk, warn, slot_cloth,
@@ -885,11 +885,12 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   // offsets[0, nt): pair count of each tile of kTravBlock leaves; offsets[nt, 2nt): the
   // tiles' exclusive prefixes (2 * nt <= t)
   const int nt = (t + kTravBlock - 1) / kTravBlock;
-  const int round = kXcds * kTravChunk;
+  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 2;
+  const int round = kXcds * xchunk;
   hipLaunchKernelGGL(k_traverse, dim3((nt + round - 1) / round * round), dim3(kTravBlock), 0, s, t,
                      b.nodes, b.amin, b.amax,
                      cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
-                     b.ncloth ? b.croot : nullptr, b.ccap, offsets);
+                     b.ncloth ? b.croot : nullptr, b.ccap, offsets, xchunk);
   CM_LAUNCH_CHECK();
   int rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
                       nullptr, s);

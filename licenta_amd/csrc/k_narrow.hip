@@ -934,7 +934,8 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   hipLaunchKernelGGL(k_ccd_plan, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
                      nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, nb.cplan, nb.ccp,
                      nb.d_ntotal);
-  hipLaunchKernelGGL(k_ccd_secant, dim3(kSecantGrid), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
+  hipLaunchKernelGGL(k_ccd_secant, dim3(nb.secant_grid > 0 ? nb.secant_grid : kSecantGrid),
+                     dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
                      nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, x, v, nb.croot);
   hipLaunchKernelGGL(k_ccd_near, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp,
