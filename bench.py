@@ -76,6 +76,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=150)
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="1: pick the single cloth's launch shapes with cm_autotune after the "
+                         "warm-up (tuned on the early, flat state; off by default because "
+                         "the late state prefers other shapes); the batched leg always tunes")
     ap.add_argument("--batch-cloths", type=int, default=512,
                     help="cloths of the batched-vs-plain leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=30)
@@ -109,6 +113,9 @@ def main():
     scene = configs.config_scene(1, args.size)
     body = ClothBody.from_scene(scene)
     body.fixed_update(args.warmup)
+    body.sync()
+    tuned = body.autotune(3) if args.autotune else None  # launch shapes only
+    body.fixed_update(1)  # captures the substep graph (again, after tuning)
     body.sync()
     t0 = time.perf_counter()
     body.fixed_update(args.steps)
@@ -167,6 +174,7 @@ def main():
         substep_roofline_frac=per_substep / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         last_substep=dict(pairs=counts["pairs"], vf=counts["vf"], ee=counts["ee"],
                           warnings=warn),
+        autotuned=tuned,
         stage_ms_after_run=stage_ms,
     )
     body.close()

@@ -902,7 +902,7 @@ __global__ void k_copy_i32(const int* __restrict__ s, int* __restrict__ d, int n
 
 }  // namespace
 
-constexpr int kSecantGrid = 2048;  // persistent workgroups of the secant pass
+constexpr int kSecantGrid = 4096;  // persistent workgroups of the secant pass (default)
 
 static int grid_for(int64_t cap) {
   int64_t g = (cap + kBlock - 1) / kBlock;
