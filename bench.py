@@ -114,9 +114,11 @@ def main():
     body = ClothBody.from_scene(scene)
     body.fixed_update(args.warmup)
     body.sync()
-    tuned = body.autotune(3) if args.autotune else None  # launch shapes only
-    body.fixed_update(1)  # captures the substep graph (again, after tuning)
-    body.sync()
+    tuned = None
+    if args.autotune:
+        tuned = body.autotune(3)  # launch shapes only
+        body.fixed_update(1)  # captures the tuned substep graph
+        body.sync()
     t0 = time.perf_counter()
     body.fixed_update(args.steps)
     body.sync()
