@@ -1271,6 +1271,7 @@ int adj_setup(cm_body* b, int nsub, int every) {
       (rc = adj_alloc(b, &a.ctl, 2 + (size_t)every)) || (rc = adj_alloc(b, &a.ab.lx, n)) ||
       (rc = adj_alloc(b, &a.ab.lv, n)) || (rc = adj_alloc(b, &a.ab.lp, n)) ||
       (rc = adj_alloc(b, &a.ab.la, n)) || (rc = adj_alloc(b, &a.ab.av, n)) ||
+      (rc = adj_alloc(b, &a.ab.dcnt, (size_t)I * n)) ||
       (rc = adj_alloc(b, &a.ab.gks, (size_t)b->S)) || (rc = adj_alloc(b, &a.ab.gkb, (size_t)b->B)))
     return rc;
   a.arena_cap = (int64_t)every * std::max<int64_t>(b->n, 65536);

@@ -223,6 +223,7 @@ struct AdjBufs {
   float4 *lx = nullptr, *lv = nullptr;  // dL/dx, dL/dv (n)
   float4 *lp = nullptr, *la = nullptr;  // dL/dp of the iterates (n each, alternating)
   float4* av = nullptr;                 // lp / count per vertex (n)
+  int* dcnt = nullptr;                  // non-zero contact records per iteration and vertex (iters * n)
   double *gks = nullptr, *gkb = nullptr;  // dL/dkp per distance / bend constraint
   const int* ecid = nullptr;              // per tile entry: constraint index (bit 31: bend)
 };

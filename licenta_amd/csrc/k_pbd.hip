@@ -726,25 +726,41 @@ __global__ void __launch_bounds__(kBlock) k_adj_predict(int n, float4* __restric
   lv[i] = f4(mul(g, dt), 0.f);
 }
 
-// per vertex: the iteration's count at positions ps (static slots plus the dynamic
-// records whose correction is non-zero, as k_solve counts them) and av = lp / count
-__global__ void __launch_bounds__(kBlock) k_adj_scale(int n, const int* __restrict__ vprefix,
-                                                      const int* __restrict__ dprefix,
-                                                      const RecPayload* __restrict__ rec,
-                                                      const float4* __restrict__ ps,
-                                                      const float4* __restrict__ lp,
-                                                      float4* __restrict__ av, float th) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  int cnt = vprefix[i + 1] - vprefix[i];
-  if (dprefix) {
-    const int e = dprefix[i + 1];
-    for (int k = dprefix[i]; k < e; ++k) {
-      const RecPayload rp = rec[k];
-      const float4 P[4] = {ps[rp.ids.x & 0x1fffffff], ps[rp.ids.y], ps[rp.ids.z], ps[rp.ids.w]};
-      if (!is_zero(rec_corr(rp, P, th))) ++cnt;
+// The dynamic part of every iteration's Jacobi counts, all iterations at once (the
+// substep's iterates are on the tape): record q adds one to dcnt[it * n + v], v its
+// vertex, when its correction at the iterate pk[it] is non-zero, as k_solve counts it.
+// Integer sums, so the order of the atomics does not matter.
+__global__ void __launch_bounds__(kBlock) k_adj_counts(int n, int iters,
+                                                       const int* __restrict__ dprefix,
+                                                       const RecPayload* __restrict__ rec,
+                                                       const float4* __restrict__ pk,
+                                                       int* __restrict__ dcnt, float th) {
+  const int nrec = dprefix[n];
+  const int64_t total = (int64_t)nrec * iters;
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * kBlock) {
+    const int it = (int)(k / nrec), q = (int)(k - (int64_t)it * nrec);
+    const RecPayload rp = rec[q];
+    const float4* ps = pk + (int64_t)it * n;
+    const int role = (rp.ids.x >> 29) & 3;
+    const int id0 = rp.ids.x & 0x1fffffff;
+    const float4 P[4] = {ps[id0], ps[rp.ids.y], ps[rp.ids.z], ps[rp.ids.w]};
+    if (!is_zero(rec_corr(rp, P, th))) {
+      const int v = role == 0 ? id0 : (role == 1 ? rp.ids.y : (role == 2 ? rp.ids.z : rp.ids.w));
+      atomicAdd(&dcnt[(int64_t)it * n + v], 1);
     }
   }
+}
+
+// per vertex: the iteration's count (static slots plus the non-zero dynamic records
+// counted by k_adj_counts; dcnt null: no contacts) and av = lp / count
+__global__ void __launch_bounds__(kBlock) k_adj_scale(int n, const int* __restrict__ vprefix,
+                                                      const int* __restrict__ dcnt,
+                                                      const float4* __restrict__ lp,
+                                                      float4* __restrict__ av) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int cnt = vprefix[i + 1] - vprefix[i] + (dcnt ? dcnt[i] : 0);
   av[i] = cnt ? f4(divs(xyz(lp[i]), (float)cnt), 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
@@ -1177,12 +1193,17 @@ int launch_adj_substep(const PbdBufs& b, const PbdParams& pp, const AdjBufs& ab,
   a.rec = rec;
   a.gks = ab.gks;
   a.gkb = ab.gkb;
+  if (dprefix) {
+    CM_HIP(hipMemsetAsync(ab.dcnt, 0, sizeof(int) * (size_t)pp.iters * b.n, s));
+    hipLaunchKernelGGL(k_adj_counts, dim3(1024), dim3(kBlock), 0, s, b.n, pp.iters, dprefix, rec,
+                       pk, ab.dcnt, pp.th);
+  }
   float4* src = ab.lp;
   float4* dst = ab.la;
   for (int it = pp.iters - 1; it >= 0; --it) {
     const float4* ps = pk + (int64_t)it * b.n;
-    hipLaunchKernelGGL(k_adj_scale, dim3(g), dim3(kBlock), 0, s, b.n, b.vprefix, dprefix, rec, ps,
-                       src, ab.av, pp.th);
+    hipLaunchKernelGGL(k_adj_scale, dim3(g), dim3(kBlock), 0, s, b.n, b.vprefix,
+                       dprefix ? ab.dcnt + (int64_t)it * b.n : nullptr, src, ab.av);
     hipLaunchKernelGGL(k_adj_iter, dim3(b.ntiles), dim3(kBlock), 0, s, a, ps, ab.av, src, dst);
     std::swap(src, dst);
   }
