@@ -487,10 +487,13 @@ template <bool kTiles>
 __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __restrict__ pin,
                                                   float4* __restrict__ pout) {
   __shared__ float cx[kLds], cy[kLds], cz[kLds];
-  int v0, v1;
+  int v0, v1, tile = 0;
   if (kTiles) {
-    v0 = a.tstart[blockIdx.x];
-    v1 = a.tstart[blockIdx.x + 1];
+    // each XCD takes a contiguous range of tiles, so the positions that neighbouring
+    // tiles share are fetched into one L2
+    tile = xcd_block(blockIdx.x, gridDim.x);
+    v0 = a.tstart[tile];
+    v1 = a.tstart[tile + 1];
   } else {
     v0 = blockIdx.x * kSolveV;
     v1 = min(v0 + kSolveV, a.n);
@@ -539,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) k_solve(SolveArgs a, const float4* __r
     for (int c0 = s0; c0 < s1; c0 += kLds) {
       const int c1 = min(c0 + kLds, s1);
       if (kTiles) {  // a tile's slots fit one pass (c0 == s0)
-        const int eb = a.eprefix[blockIdx.x], ee = a.eprefix[blockIdx.x + 1];
+        const int eb = a.eprefix[tile], ee = a.eprefix[tile + 1];
         // kBatch entries per thread at a time: their loads, then their positions'
         // loads, are in flight together before any of them is evaluated
         for (int q0 = eb + tid; q0 < ee; q0 += kBlock * kBatch) {
@@ -947,14 +950,15 @@ __global__ void __launch_bounds__(kBlock) k_adj_iter(AdjArgs a, const float4* __
                                                      const float4* __restrict__ lp,
                                                      float4* __restrict__ la) {
   __shared__ float cx[kLds], cy[kLds], cz[kLds];
-  const int v0 = a.tstart[blockIdx.x], v1 = a.tstart[blockIdx.x + 1];
+  const int tile = xcd_block(blockIdx.x, gridDim.x);  // contiguous tiles per XCD
+  const int v0 = a.tstart[tile], v1 = a.tstart[tile + 1];
   const int tid = threadIdx.x;
   const int i = v0 + tid;
   const bool mine = i < v1;
   f3 total = mk3(0, 0, 0);
   {
     const int s0 = a.vprefix[v0];
-    const int eb = a.eprefix[blockIdx.x], ee = a.eprefix[blockIdx.x + 1];
+    const int eb = a.eprefix[tile], ee = a.eprefix[tile + 1];
     for (int q = eb + tid; q < ee; q += kBlock) {
       const TileEntry t = a.entries[q];
       const int cid = a.ecid[q];
