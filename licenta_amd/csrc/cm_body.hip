@@ -1509,12 +1509,19 @@ int set_tiles(cm_body* b, int tile_v) {
   free_alloc(b, b->eprefix);
   free_alloc(b, b->entries);
   free_alloc(b, b->adj.ecid);
+  // until the new tiles are in place the body uses the slot-parallel Jacobi form,
+  // which needs no tiles (and stays so if an upload fails)
+  b->ntiles = 0;
+  b->tstart = b->eprefix = nullptr;
+  b->entries = nullptr;
+  b->adj.ecid = nullptr;
+  b->adj.ab.ecid = nullptr;
+  b->adj.ready = false;
   int rc;
   if ((rc = upload(b, &b->tstart, h.tstart)) || (rc = upload(b, &b->eprefix, h.eprefix)) ||
       (rc = upload(b, &b->entries, h.entries)) || (rc = upload(b, &b->adj.ecid, h.ecid)))
     return rc;
   b->adj.ab.ecid = b->adj.ecid;
-  b->adj.ready = false;
   b->ntiles = (int)h.tstart.size() - 1;
   b->tile_v = tile_v;
   return CM_OK;
