@@ -56,3 +56,16 @@ def test_create_fails_cleanly_on_bad_input():
                        None, None, C.byref(h))
     assert rc == -1 and h.value is None
     assert b"out of range" in lib.cm_last_error()
+
+
+def test_entry_points_reject_null_body():
+    """Every body entry point checks its handle before touching a device."""
+    lib = _abi.lib()
+    buf = np.zeros(12, np.float32)
+    picks = np.zeros(3, np.int32)
+    assert lib.cm_step(None, 1) == -1
+    assert lib.cm_set_stiffness(None, None, None) == -1
+    assert lib.cm_forward_ckpt(None, 4, 2) == -1
+    assert lib.cm_adjoint(None, _abi.ptr(buf), None, None, None, None, None) == -1
+    assert lib.cm_autotune(None, 2, _abi.ptr(picks)) == -1
+    assert b"null body" in lib.cm_last_error()
