@@ -817,7 +817,8 @@ constexpr int kShortList = 32;   // longer record lists are ordered by a workgro
 constexpr int kLongLds = 4096;   // records a workgroup ranks in LDS
 
 // per vertex: order its records by record index. Short lists: insertion sort by
-// the vertex's thread; longer ones go to a work list for k_dyn_sort_long.
+// the vertex's thread in its own LDS column (one load per record, then LDS only);
+// longer ones go to a work list for k_dyn_sort_long.
 __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restrict__ prefix,
                                                      uint32_t* __restrict__ drec,
                                                      int* __restrict__ nlong,
@@ -828,24 +829,32 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restric
                                                      const int4* __restrict__ ee_v,
                                                      const float4* __restrict__ ee_w,
                                                      RecPayload* __restrict__ rec) {
-  const int v = blockIdx.x * kBlock + threadIdx.x;
+  __shared__ uint32_t sk[kShortList][kBlock];  // column per thread: no bank conflicts
+  const int tid = threadIdx.x;
+  const int v = blockIdx.x * kBlock + tid;
   if (v >= n) return;
   const int b = prefix[v], e = prefix[v + 1];
-  if (e - b > kShortList) {
+  const int k = e - b;
+  if (k > kShortList) {
     long_list[atomicAdd(nlong, 1)] = v;
     return;
   }
-  for (int i = b + 1; i < e; ++i) {
-    const uint32_t key = drec[i];
+  for (int j = 0; j < k; ++j) sk[j][tid] = drec[b + j];
+  for (int i = 1; i < k; ++i) {
+    const uint32_t key = sk[i][tid];
     int j = i - 1;
-    while (j >= b && drec[j] > key) {
-      drec[j + 1] = drec[j];
+    while (j >= 0 && sk[j][tid] > key) {
+      sk[j + 1][tid] = sk[j][tid];
       --j;
     }
-    drec[j + 1] = key;
+    sk[j + 1][tid] = key;
   }
   const int nvf = *d_nvf;
-  for (int i = b; i < e; ++i) put_payload(drec[i], i, nvf, vf_v, vf_w, ee_v, ee_w, rec);
+  for (int j = 0; j < k; ++j) {
+    const uint32_t r = sk[j][tid];
+    drec[b + j] = r;
+    put_payload(r, b + j, nvf, vf_v, vf_w, ee_v, ee_w, rec);
+  }
 }
 
 // one workgroup per long list: record indices are unique, so the rank of a record
