@@ -1,6 +1,7 @@
 """Scene builders: grid meshes and constraint sets have the sizes the workloads
-name (Appendix-D counts of the survey: 64^2 -> 7,938 triangles, 16,002
-structural+shear constraints; 256^2 -> 130,050 and 260,610)."""
+name (an n x n grid split along one diagonal has 2 (n-1)^2 triangles and
+2 n (n-1) + 2 (n-1)^2 structural+shear constraints: 64^2 -> 7,938 and 16,002;
+256^2 -> 130,050 and 260,610)."""
 import numpy as np
 
 from licenta_amd import configs
