@@ -117,6 +117,10 @@ typedef struct cm_params {
   int32_t pair_cap;        /* per-leaf triangle-pair cap (reference 64)          */
   uint32_t flags;          /* CM_F_*                                             */
   uint32_t ref_compat;     /* CM_RC_*                                            */
+  /* capacity overrides, 0 = default: [0] / [1] VF / EE candidates per triangle
+   * (128 / 192), [2] CCD survivors per triangle (16), [3] contact records per
+   * substep that cm_adjoint's tape holds at first (max(N, 65536); it grows when a
+   * segment needs more); [4..6] unused */
   int32_t reserved[7];
 } cm_params;
 

@@ -53,7 +53,7 @@ assert CONTACT_DTYPE.itemsize == 48 == C.sizeof(CmContact)
 def make_params(d: dict) -> CmParams:
     p = CmParams()
     for name, typ in CmParams._fields_:
-        if name == "reserved":  # [0..2]: VF / EE candidates and contacts per triangle
+        if name == "reserved":  # capacity overrides (see cm_params in clothmi.h)
             for i, x in enumerate(d.get("reserved", ())):
                 p.reserved[i] = int(x)
             continue

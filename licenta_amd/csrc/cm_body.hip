@@ -1274,7 +1274,8 @@ int adj_setup(cm_body* b, int nsub, int every) {
       (rc = adj_alloc(b, &a.ab.dcnt, (size_t)I * n)) ||
       (rc = adj_alloc(b, &a.ab.gks, (size_t)b->S)) || (rc = adj_alloc(b, &a.ab.gkb, (size_t)b->B)))
     return rc;
-  a.arena_cap = (int64_t)every * std::max<int64_t>(b->n, 65536);
+  const int64_t per = b->prm.reserved[3] > 0 ? b->prm.reserved[3] : std::max<int64_t>(b->n, 65536);
+  a.arena_cap = (int64_t)every * per;
   if ((rc = adj_alloc(b, &a.arena, (size_t)a.arena_cap))) return rc;
   a.every = every;
   a.nck = nck;

@@ -233,3 +233,23 @@ def test_batched_adjoint_matches_single_bodies():
         assert scaled_err(gkd[s0[c]:s0[c + 1]], skd) <= 1e-5
         assert scaled_err(gkb[b0[c]:b0[c + 1]], skb) <= 1e-5
     batch.close()
+
+
+def test_contact_tape_growth():
+    """With a contact tape too small for a segment (one record per substep at first),
+    cm_adjoint grows it and recomputes the segment: the gradients equal those of a
+    body whose tape never overflows."""
+    from licenta_amd.body import ClothBody
+    print("adjoint tape growth size=8", flush=True)
+    sc, body, kd, kb, v0, target = _problem(8, selfcoll=True)
+    assert sum(a + b for a, b in _gpu_counts(body, sc, v0, 4)) > 0  # contacts occur
+    g_ref, _ = _gpu_grads(body, target, 4, 2)
+    body.close()
+    small = dict(sc, params=dict(sc["params"], reserved=(0, 0, 0, 1, 0, 0, 0)))
+    b = ClothBody.from_scene(small)
+    b.set_stiffness(kd, kb)
+    b.set_state(sc["xyz"], v0)
+    g, _ = _gpu_grads(b, target, 4, 2)
+    for k in ("kd", "kb", "x", "v"):
+        assert np.array_equal(g[k], g_ref[k]), k
+    b.close()
