@@ -75,6 +75,8 @@ extern "C" {
 #define CM_F_SPHERE (1u << 5)         /* static/moving analytic sphere          */
 #define CM_F_EE_CONTACTS (1u << 6)    /* edge-edge contacts in the solver       */
 #define CM_F_FRICTION (1u << 7)       /* velocity friction after integration    */
+/* (ground contacts, then the substep's VF / EE contacts; coefficient k_friction;
+ * Physics/CudaPBD.cu:770-833). Not with cm_create_batch or cm_forward_ckpt. */
 /* Evaluate the dihedral bend (Physics/CudaPBD.cu:1024-1147) in FP64 from the FP32
  * positions and round the corrections to FP32. On absolute positions the
  * reference formula cancels catastrophically in FP32 wherever two vertices are
@@ -92,7 +94,12 @@ extern "C" {
 /* FindSplit returns (first+last)>>1 when the end codes are equal
  * (Collision/CudaBVH.cu:219-220); off = index-augmented Karras split. */
 #define CM_RC_MIDPOINT_SPLIT (1u << 2)
-#define CM_RC_ALL (CM_RC_STALE_LEAF_AABB | CM_RC_BEND_ABS | CM_RC_MIDPOINT_SPLIT)
+/* The ground step of the friction pass reaches only vertices below
+ * 255 * ceil(N/256): the reference launches it with 255-thread blocks
+ * (Physics/CudaPBD.cu:778); off = every vertex. */
+#define CM_RC_RIGID_FRICTION_TAIL (1u << 3)
+#define CM_RC_ALL \
+  (CM_RC_STALE_LEAF_AABB | CM_RC_BEND_ABS | CM_RC_MIDPOINT_SPLIT | CM_RC_RIGID_FRICTION_TAIL)
 
 /* POD replacement for Physics::ClothParams (Physics/Structs.h:23-61). */
 typedef struct cm_params {
@@ -177,8 +184,8 @@ int cm_destroy(cm_body* body);
  * per cloth; k_stretch / k_bend give one stiffness per cloth (NULL: params). Each
  * cloth has its own LBVH (30-bit codes in the cloth's centroid box; cloths never
  * collide with each other). Positions and buffers are read back for all vertices,
- * cloth after cloth. Requires morton_bits 30 with morton_normalize, and no LRA or
- * rigid-mode damping. */
+ * cloth after cloth. Requires morton_bits 30 with morton_normalize, and no LRA,
+ * rigid-mode damping or friction. */
 int cm_create_batch(const cm_params* params, int32_t ncloth, const int32_t* cloth_n,
                     const int32_t* cloth_t, const int32_t* cloth_ndist, const float* xyz,
                     const uint32_t* tri, const uint8_t* pinned, const float* inv_mass,

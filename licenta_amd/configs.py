@@ -37,7 +37,8 @@ F_BEND_FP64 = 1 << 8
 RC_STALE_LEAF_AABB = 1 << 0
 RC_BEND_ABS = 1 << 1
 RC_MIDPOINT_SPLIT = 1 << 2
-RC_ALL = RC_STALE_LEAF_AABB | RC_BEND_ABS | RC_MIDPOINT_SPLIT
+RC_RIGID_FRICTION_TAIL = 1 << 3
+RC_ALL = RC_STALE_LEAF_AABB | RC_BEND_ABS | RC_MIDPOINT_SPLIT | RC_RIGID_FRICTION_TAIL
 
 
 def default_params() -> dict:

@@ -428,6 +428,7 @@ PbdParams pbd_params(const cm_body* b) {
   pp.sr = P.sphere_radius;
   pp.kg = P.global_vel_damp;
   pp.kd = P.k_damp;
+  pp.kf = P.k_friction;
   return pp;
 }
 
@@ -511,6 +512,7 @@ int record_substep(cm_body* b, hipEvent_t* ev = nullptr, const TapeSlot* tape = 
   mark(CM_STAGE_INTEGRATE);
   rc = tape ? launch_integrate_from(pb, pp, tape->pk + (int64_t)pp.iters * b->n, s)
             : launch_integrate(pb, pp, s);
+  if (!rc && (P.flags & CM_F_FRICTION)) rc = launch_friction(pb, pp, nb, s);
   mark(CM_NUM_STAGES);
   return rc;
 }
@@ -840,9 +842,9 @@ int cm_create_batch(const cm_params* params, int32_t ncloth, const int32_t* clot
     return CM_ERR_ARG;
   }
   if (params->morton_bits != 30 || !params->morton_normalize ||
-      (params->flags & (CM_F_LRA | CM_F_DAMPING)) || ncloth > (1 << 20)) {
-    set_error("cm_create_batch: a batch needs 30-bit keys in each cloth's box, no LRA and no "
-              "rigid-mode damping, and at most 2^20 cloths");
+      (params->flags & (CM_F_LRA | CM_F_DAMPING | CM_F_FRICTION)) || ncloth > (1 << 20)) {
+    set_error("cm_create_batch: a batch needs 30-bit keys in each cloth's box, no LRA, no "
+              "rigid-mode damping, no friction, and at most 2^20 cloths");
     return CM_ERR_ARG;
   }
   BatchSpec bs;

@@ -204,7 +204,7 @@ struct PbdParams {
   f3 wmin, wmax;
   f3 sc;
   float sr;
-  float kg, kd;
+  float kg, kd, kf;
 };
 int launch_predict(const PbdBufs& b, const PbdParams& pp, hipStream_t s);
 int launch_solver(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, hipStream_t s);
@@ -213,6 +213,7 @@ int launch_solve_only(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& n
 int launch_integrate(const PbdBufs& b, const PbdParams& pp, hipStream_t s);
 int launch_integrate_from(const PbdBufs& b, const PbdParams& pp, const float4* pfin,
                           hipStream_t s);
+int launch_friction(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, hipStream_t s);
 // the solver stage with every iterate kept: pk[k * n, (k + 1) * n) = positions
 // entering iteration k, pk[iters * n ...) the result
 int launch_solver_rec(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, float4* pk,

@@ -38,6 +38,13 @@ CM_HD f3 vmax(f3 a, f3 b) { return f3{fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.
 CM_HD f3 min3(f3 a, f3 b, f3 c) { return vmin(a, vmin(b, c)); }
 CM_HD f3 max3(f3 a, f3 b, f3 c) { return vmax(a, vmax(b, c)); }
 CM_HD bool is_zero(f3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+// mirror image of a about the plane with unit normal nn (Core/CudaUtils.cu:262-265)
+CM_HD f3 reflect(f3 a, f3 nn) { return sub(a, mul(nn, 2.f * dot(a, nn))); }
+// a = tangential + normal part along unit nn (Core/CudaUtils.cu:267-271)
+CM_HD void tangent_normal(f3 a, f3 nn, f3& tang, f3& norm) {
+  norm = mul(nn, dot(a, nn));
+  tang = sub(a, norm);
+}
 CM_HD float fclamp(float v, float lo, float hi) { return fminf(fmaxf(lo, v), hi); }
 CM_HD bool between(float v, float lo, float hi) { return v >= lo && v <= hi; }
 

@@ -17,6 +17,9 @@
 //                    reference's gathers (:1417-1475) without the global slot buffer.
 //  k_integrate    <- _FinalUpdate (:1916-1932) + _WorldConstraints
 //                    (Physics/CudaDeformableUtils.cu:260-269)
+//  k_friction     <- CudaPBD::ApplyFriction (:770-833): _DampVelRigid (:1879-1912),
+//                    then _ApplyFriction / _AccumulateFriction / FinalFrictionStep
+//                    (:1725-1877, :1700-1723) over the per-vertex contact records
 //
 // Positions are float4 with the inverse mass in .w, so one 16-byte load gives a
 // neighbour's position and weight.
@@ -695,6 +698,57 @@ __global__ void __launch_bounds__(kBlock) k_integrate(int n, const float4* __res
 }
 
 
+// Friction on the integrated velocities. First the ground step of _DampVelRigid
+// (:1879-1912): a vertex whose solver set-up flagged a ground contact keeps its
+// tangential velocity and has its normal velocity reflected and scaled by 1 - kf
+// (ground normal +y). The reference launches that kernel with 255-thread blocks over
+// ceil(N/256) blocks (:778), so it reaches only vertices below rigid_n; its sphere
+// branch never runs, since the sphere set-up leaves every flag clear (:1531, :1538).
+// Then the contacts: the reference writes each contact's normal once per contact
+// vertex, sorts the entries by vertex (stable) and sums each vertex's run in entry
+// order (:1745-1795, :1846-1877). The vertex's record list here is that run in the
+// same order (records ascend by contact, VF before EE), so the sum is the same. The
+// reference leaves the vertex ids of its EE entries unwritten (:1810-1813), so they
+// land on whatever ids the buffer held before; here an EE contact counts for its own
+// four vertices. With the sum's direction fn and count c, FinalFrictionStep
+// (:1700-1723) keeps the tangential velocity along fn and reflects the normal part
+// scaled by (1 - kf)^c.
+__global__ void __launch_bounds__(kBlock) k_friction(
+    int n, int rigid_n, const float4* __restrict__ ground, const int* __restrict__ dprefix,
+    const uint32_t* __restrict__ drec, const int* __restrict__ d_nvf,
+    const float4* __restrict__ vf_tn, const float4* __restrict__ ee_tn, float4* __restrict__ v,
+    float kf) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 vi = v[i];
+  f3 vel = xyz(vi);
+  if (ground && i < rigid_n && ground[i].w != 0.f) {
+    const f3 nc = mk3(0.f, 1.f, 0.f);
+    f3 vt, vn;
+    tangent_normal(vel, nc, vt, vn);
+    vel = reflect(add(vt, mul(vn, 1.f - kf)), nc);
+  }
+  if (dprefix) {
+    const int b = dprefix[i], e = dprefix[i + 1];
+    if (e > b) {
+      const int nvf = *d_nvf;
+      f3 acc = mk3(0.f, 0.f, 0.f);
+      for (int k = b; k < e; ++k) {
+        const int c = (int)(drec[k] >> 2);
+        const float4 tn = c < nvf ? vf_tn[c] : ee_tn[c - nvf];
+        acc = add(acc, mk3(tn.y, tn.z, tn.w));
+      }
+      if (!is_zero(acc)) {
+        const f3 fn = normalize(acc);
+        f3 vt, vn;
+        tangent_normal(vel, fn, vt, vn);
+        vel = reflect(add(mul(vn, powf(1.f - kf, (float)(e - b))), vt), fn);
+      }
+    }
+  }
+  v[i] = f4(vel, vi.w);
+}
+
 // ---- adjoint of the differentiable substep (no reference counterpart) ------------
 // The substep as a function of (x, v, kp): v' = v + dt g, p0 = x + dt v', I Jacobi
 // iterations p_{k+1} = p_k + (sum of the corrections at p_k) / count_k over the
@@ -1136,6 +1190,19 @@ int launch_integrate_from(const PbdBufs& b, const PbdParams& pp, const float4* p
   hipLaunchKernelGGL(k_integrate, dim3(nblocks(b.n)), dim3(kBlock), 0, s, b.n, pfin, b.x, b.v,
                      b.p, b.pinned, pp.dt, (pp.flags & CM_F_WORLD_CLAMP) ? 1 : 0, pp.wmin,
                      pp.wmax);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_friction(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, hipStream_t s) {
+  const int g = nblocks(b.n);
+  const int rigid_n = (pp.ref_compat & CM_RC_RIGID_FRICTION_TAIL) ? std::min(b.n, g * (kBlock - 1))
+                                                                   : b.n;
+  const float4* ground = (pp.flags & CM_F_GROUND) ? b.ground : nullptr;
+  const bool coll = (pp.flags & CM_F_SELF_COLLISION) && nb.dprefix;
+  hipLaunchKernelGGL(k_friction, dim3(g), dim3(kBlock), 0, s, b.n, rigid_n, ground,
+                     coll ? nb.dprefix : nullptr, nb.drec, nb.d_nvf, nb.vf_tn, nb.ee_tn, b.v,
+                     pp.kf);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

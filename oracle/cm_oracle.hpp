@@ -892,6 +892,7 @@ struct Sim {
   };
   bool record = false;
   std::vector<Tape> tape;
+  std::vector<uint8_t> gflag;  // ground contacts of the last solver set-up
   bool sphere_on = false;
   F3 sphere_c{0, 0, 0};
   float sphere_r = 0.f;
@@ -1279,7 +1280,7 @@ struct Sim {
   void solve() {
     const R th = R(prm.thickness);
     // ground (Physics/CudaPBD.cu:1477-1507)
-    std::vector<uint8_t> gflag(n, 0);
+    gflag.assign(n, 0);
     std::vector<RV> gqc(n), gnc(n);
     if (prm.flags & CM_F_GROUND) {
       const R gh = R(prm.ground_height);
@@ -1451,6 +1452,46 @@ struct Sim {
     }
   }
 
+  // ---- stage 8: friction on the integrated velocities (Physics/CudaPBD.cu:770-833,
+  // called at Physics/CudaDeformableBody.cpp:596-602) ----
+  void friction() {
+    const R kf = R(prm.k_friction);
+    // _DampVelRigid (Physics/CudaPBD.cu:1879-1912) on the ground flags; its launch of
+    // 255-thread blocks over ceil(N/256) blocks (:778) stops short of the last
+    // ceil(N/256) vertices. Its sphere branch reads flags the sphere set-up never sets
+    // (:1531, :1538).
+    const int rigid_n =
+        (prm.ref_compat & CM_RC_RIGID_FRICTION_TAIL) ? std::min(n, (n + 255) / 256 * 255) : n;
+    const RV up = mk<R>(0, 1, 0);
+    for (int i = 0; i < rigid_n; ++i) {
+      if (gflag.empty() || !gflag[i]) continue;
+      const RV vn = mul(up, dot(v[i], up));  // CudaUtils::tangentNormal
+      const RV a = (v[i] - vn) + mul(vn, R(1) - kf);
+      v[i] = a - mul(up, R(2) * dot(a, up));  // CudaUtils::reflect
+    }
+    // _ApplyFriction (:1725-1743): the contact normal once per contact vertex, VF
+    // contacts then EE; sorted by vertex (stable) and summed per vertex in entry
+    // order (:1846-1877). The EE entries' vertex ids are the contact's own here; the
+    // reference leaves them unwritten (:1810-1813).
+    std::vector<RV> acc(n, mk<R>(0, 0, 0));
+    std::vector<int> cnt(n, 0);
+    for (const std::vector<cm_contact>* cs : {&vf, &ee})
+      for (const cm_contact& c : *cs)
+        for (int k = 0; k < 4; ++k) {
+          acc[c.v[k]] = acc[c.v[k]] + mk<R>(c.n[0], c.n[1], c.n[2]);
+          ++cnt[c.v[k]];
+        }
+    // FinalFrictionStep (:1700-1723)
+    for (int i = 0; i < n; ++i) {
+      const RV s = acc[i];
+      if (cnt[i] == 0 || (s.x == R(0) && s.y == R(0) && s.z == R(0))) continue;
+      const RV fn = normalize(s);
+      const RV vn = mul(fn, dot(v[i], fn));
+      const RV a = mul(vn, R(std::pow(R(1) - kf, R(cnt[i])))) + (v[i] - vn);
+      v[i] = a - mul(fn, R(2) * dot(a, fn));
+    }
+  }
+
   // One substep (Physics/CudaDeformableBody.cpp:550-627).
   void step() {
     if (record) {
@@ -1480,6 +1521,7 @@ struct Sim {
     }
     solve();
     integrate();
+    if (prm.flags & CM_F_FRICTION) friction();
   }
 
   // ---- adjoint of the recorded rollout (no reference counterpart) ----

@@ -102,9 +102,10 @@ def test_batch_rejects_unsupported_modes():
     scenes = _scenes()[:2]
     with pytest.raises(RuntimeError):
         ClothBody.batch(scenes, dict(scenes[0]["params"], morton_bits=63))
-    with pytest.raises(RuntimeError):
-        ClothBody.batch(scenes, dict(scenes[0]["params"], flags=scenes[0]["params"]["flags"] |
-                                     configs.F_LRA))
+    for flag in (configs.F_LRA, configs.F_DAMPING, configs.F_FRICTION):
+        with pytest.raises(RuntimeError):
+            ClothBody.batch(scenes, dict(scenes[0]["params"],
+                                         flags=scenes[0]["params"]["flags"] | flag))
 
 
 def test_autotune_keeps_results():

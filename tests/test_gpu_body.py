@@ -171,6 +171,58 @@ def test_folded_cloth_contacts(oracle_mod):
     assert total > 0
 
 
+@pytest.mark.parametrize("tail", [True, False])
+def test_friction_folded_on_ground(oracle_mod, tail):
+    """Friction (ground step, then the substep's VF / EE contacts) on a 16 x 16 cloth
+    folded onto itself on the ground: 256 vertices, so the reference's 255-thread
+    ground step misses the last one when that behaviour is on. Friction changes only
+    the velocities, which each substep here compares with the oracle's (relative to
+    the largest speed) besides everything compare_substep checks."""
+    n = 16
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    h = n // 2
+    for i in range(h, n):
+        src = 2 * h - 1 - i
+        xyz[i, :, 0] = xyz[src, :, 0] + 0.002
+        xyz[i, :, 1] = 0.003 + 0.0005 * (i - h)
+    sc["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    rc = configs.RC_ALL if tail else configs.RC_ALL & ~configs.RC_RIGID_FRICTION_TAIL
+    sc["params"] = dict(sc["params"], thickness=0.004, k_friction=0.25, ref_compat=rc,
+                        flags=sc["params"]["flags"] | configs.F_GROUND | configs.F_FRICTION)
+    sc["pinned"] = None
+    sc["inv_mass"] = np.full(n * n, 1000.0, np.float32)
+    from licenta_amd.body import ClothBody
+    body = ClothBody.from_scene(sc)
+    rng = np.random.default_rng(11)
+    v = rng.normal(0, 0.05, (n * n, 3)).astype(np.float32)
+    v.reshape(n, n, 3)[h:, :, 1] = -0.3
+    body.set_state(sc["xyz"], v)
+    total = 0
+    for s in range(4):
+        print(f"[friction tail={tail}] substep {s}", flush=True)
+        o32, o64 = seeded_pair(oracle_mod, sc, body)
+        body.fixed_update(1)
+        o32.step(1)
+        o64.step(1)
+        gvf, gee = body.contacts()
+        ovf, oee = o32.contacts(False), o32.contacts(True)
+        assert np.array_equal(gvf["v"], ovf["v"]) and np.array_equal(gee["v"], oee["v"])
+        total += len(gvf) + len(gee)
+        x, vg = body.positions(), body.velocities()
+        x32, _, v32 = o32.state()
+        x64, _, v64 = o64.state()
+        assert rel_err(x, x32) <= TOL32 and rel_err(x, x64) <= TOL64
+        # v = (p - x) / dt turns a position rounding difference into a larger
+        # velocity one: 1e-3 of the largest speed
+        e32, e64 = rel_err(vg, v32), rel_err(vg, v64)
+        print(f"  contacts {len(gvf)} + {len(gee)}, velocity rel err {e32:.2e} / {e64:.2e}",
+              flush=True)
+        assert e32 <= 1e-3 and e64 <= 1e-3
+    body.close()
+    assert total > 0
+
+
 def test_graph_replay_matches_eager_state(oracle_mod):
     """Several substeps in one cm_step call equal the same substeps one by one."""
     from licenta_amd.body import ClothBody

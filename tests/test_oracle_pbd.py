@@ -224,3 +224,89 @@ def test_thread_count_invariance(oracle_mod):
         for ra, rb in zip(a, b):
             for u, w in zip(ra, rb):
                 assert np.array_equal(u, w)
+
+
+def _friction_pair(oracle_mod, sc, prec=64):
+    """Two oracles on one scene, friction off and on; friction acts on the velocities
+    after integration only, so one substep from the same state gives the same
+    positions and contacts, and the velocities differ by the friction step alone."""
+    off = dict(sc, params=dict(sc["params"], flags=sc["params"]["flags"] & ~configs.F_FRICTION))
+    on = dict(sc, params=dict(sc["params"], flags=sc["params"]["flags"] | configs.F_FRICTION))
+    return oracle_mod.OracleSim(off, prec), oracle_mod.OracleSim(on, prec)
+
+
+@pytest.mark.parametrize("tail", [True, False])
+def test_friction_ground_closed_form(oracle_mod, tail):
+    """A 16x16 cloth (256 vertices) resting inside the ground's thickness: every vertex
+    has a ground contact, and the ground friction leaves (vx, -(1-kf) vy, vz) (the
+    normal part reflected and damped, the tangential part kept). With the reference's
+    255-thread launch the last vertex (index 255) is not reached."""
+    n, kf = 16, 0.3
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(-1, 3).copy()  # xz-plane grid
+    xyz[:, 1] = 0.004
+    rc = configs.RC_ALL if tail else configs.RC_ALL & ~configs.RC_RIGID_FRICTION_TAIL
+    sc = dict(sc, xyz=xyz.astype(np.float32), pinned=None,
+              inv_mass=np.ones(n * n, np.float32),
+              params=dict(sc["params"], flags=configs.F_GROUND, k_friction=kf, thickness=0.01,
+                          ground_height=0.0, k_stretch=0.0, k_bend=0.0, ref_compat=rc))
+    a, b = _friction_pair(oracle_mod, sc)
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(0, 0.05, (n * n, 3))
+    v0[:, 1] = -np.abs(v0[:, 1]) - 0.01
+    for o in (a, b):
+        o.set_state(xyz, v0)
+        o.step(1)
+    xa, _, va = a.state()
+    xb, _, vb = b.state()
+    assert np.array_equal(xa, xb)
+    assert (va[:, 1] != 0).all()
+    want = va * np.array([1.0, -(1.0 - np.float64(np.float32(kf))), 1.0])
+    reach = 255 if tail else 256
+    np.testing.assert_allclose(vb[:reach], want[:reach], rtol=1e-12, atol=1e-15)
+    assert np.array_equal(vb[reach:], va[reach:])
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_friction_contacts_invariants(oracle_mod, prec):
+    """A folded cloth with VF and EE contacts. For each vertex the contact normals of
+    its contacts sum to s with count c (from the contact lists); with f = s/|s|, the
+    friction keeps the velocity's part orthogonal to f and maps its part along f to
+    -(1-kf)^c times itself. Vertices without contacts keep their velocity."""
+    n, kf = 20, 0.25
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    xyz[n // 2:, :, 1] = 0.003
+    xyz[n // 2:, :, 0] = xyz[n - 1 - np.arange(n // 2, n), :, 0]
+    sc = dict(sc, xyz=xyz.reshape(-1, 3), params=dict(sc["params"], k_friction=kf))
+    a, b = _friction_pair(oracle_mod, sc, prec)
+    a.step(2)  # to a state with many contacts
+    x, p, v = a.state()
+    tri = a.get_i32("tri").reshape(-1, 4)
+    b.set_state(x, v, p, tri)
+    a.step(1)
+    b.step(1)
+    xa, _, va = a.state()
+    xb, _, vb = b.state()
+    assert np.array_equal(xa, xb)
+    cs = np.concatenate([a.contacts(False), a.contacts(True)])
+    assert np.array_equal(cs, np.concatenate([b.contacts(False), b.contacts(True)]))
+    assert len(cs) > 100
+    s = np.zeros((n * n, 3))
+    cnt = np.zeros(n * n, int)
+    for c in cs:
+        for k in range(4):
+            s[c["v"][k]] += c["n"].astype(np.float64)
+            cnt[c["v"][k]] += 1
+    touched = cnt > 0
+    assert touched.sum() > 20
+    assert np.array_equal(vb[~touched], va[~touched])
+    f = s[touched] / np.linalg.norm(s[touched], axis=1, keepdims=True)
+    dn_a = np.einsum("ij,ij->i", va[touched], f)
+    dn_b = np.einsum("ij,ij->i", vb[touched], f)
+    k32 = np.float64(np.float32(kf))
+    tol = 1e-4 if prec == 32 else 1e-10
+    scale = np.linalg.norm(va[touched], axis=1).max()
+    np.testing.assert_allclose(dn_b, -((1.0 - k32) ** cnt[touched]) * dn_a, rtol=0, atol=tol * scale)
+    np.testing.assert_allclose(vb[touched] - dn_b[:, None] * f, va[touched] - dn_a[:, None] * f,
+                               rtol=0, atol=tol * scale)
