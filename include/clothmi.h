@@ -15,6 +15,8 @@
  *                         (Physics/CudaDeformableBody.cpp:629-638, 473-495)
  *   cm_read_normals    <- DeformableUtils::UpdateVertexNormals + FinalVertUpdate
  *                         (Physics/CudaDeformableUtils.cu:190-249,317-344)
+ *   cm_read_vertices   <- FinalVertUpdate's VertexFormat copy to the host
+ *                         (Physics/CudaDeformableBody.cpp:473-495)
  *   cm_set_sphere      <- Physics::CudaDeformableBody::SetSpherePos
  *                         (Physics/CudaDeformableBody.cpp:640-643)
  *   cm_destroy         <- Physics::CudaDeformableBody::~CudaDeformableBody
@@ -82,6 +84,12 @@ extern "C" {
  * reference formula cancels catastrophically in FP32 wherever two vertices are
  * nearly collinear with the origin; FP64 gives the formula's exact value there. */
 #define CM_F_BEND_FP64 (1u << 8)
+/* Vertex normals every substep, as the reference's FixedUpdate computes them
+ * (DeformableUtils::UpdateVertexNormals, Physics/CudaDeformableUtils.cu:190-249):
+ * the unit face normals of the predicted positions added to the previous
+ * substep's normal, then normalised. Without it cm_read_normals computes the
+ * normals of the current positions when called. */
+#define CM_F_VERTEX_NORMALS (1u << 9)
 
 /* ---- reference-compatibility bits (cm_params.ref_compat) ----------------- */
 /* Leaf AABBs stay in pre-sort triangle slots after the Morton sort
@@ -199,6 +207,15 @@ int cm_sync(cm_body* body);
 int cm_read_positions(cm_body* body, float* xyz);   /* 3N, syncs */
 int cm_read_velocities(cm_body* body, float* xyz);  /* 3N, syncs */
 int cm_read_normals(cm_body* body, float* xyz);     /* 3N, syncs */
+/* Rendering::VertexFormat readback (Physics/CudaDeformableBody.cpp:473-495,
+ * Physics/CudaDeformableUtils.cu:317-344): verts holds N records of
+ * CM_VERTEX_FLOATS floats laid out as Rendering/VertexFormat.h:24-54 (position 3,
+ * color 4, normal 3, uv 2; 48 bytes). Writes each record's position (the carried
+ * positions) and normal (as cm_read_normals) and leaves color and uv. Syncs. */
+#define CM_VERTEX_FLOATS 12
+#define CM_VERTEX_POSITION 0
+#define CM_VERTEX_NORMAL 7
+int cm_read_vertices(cm_body* body, float* verts);
 int cm_set_sphere(cm_body* body, const float center[3], float radius);
 /* Replace the state. x, p, v: 3N floats each (p may be NULL: p = x);
  * tri: 4T ints (current triangle order with feature bits) or NULL. */

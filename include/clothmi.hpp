@@ -65,6 +65,12 @@ class ClothBodyHip : public PhysicsBody {
     check(cm_read_normals(m_body, m_normals.data()));
   }
 
+  /* Update into the caller's Rendering::VertexFormat array (N records of 48 bytes):
+   * positions and normals written, color and uv kept (cm_read_vertices). */
+  void UpdateVertices(void* vertex_format_records) {
+    check(cm_read_vertices(m_body, static_cast<float*>(vertex_format_records)));
+  }
+
   void SetSpherePos(const float center[3], float radius) {
     check(cm_set_sphere(m_body, center, radius));
   }

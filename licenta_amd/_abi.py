@@ -97,7 +97,7 @@ def lib():
     L.cm_destroy.argtypes = [vp]
     L.cm_step.argtypes = [vp, i32]
     L.cm_sync.argtypes = [vp]
-    for f in ("cm_read_positions", "cm_read_velocities", "cm_read_normals"):
+    for f in ("cm_read_positions", "cm_read_velocities", "cm_read_normals", "cm_read_vertices"):
         getattr(L, f).argtypes = [vp, vp]
     L.cm_set_sphere.argtypes = [vp, vp, C.c_float]
     L.cm_set_state.argtypes = [vp, vp, vp, vp, vp]
@@ -142,5 +142,8 @@ EXPORTS = (
     "cm_query_contacts", "cm_tri_update", "cm_lbvh_sort_temp_bytes", "cm_lbvh_sort",
     "cm_lbvh_build_temp_bytes", "cm_lbvh_build", "cm_lbvh_refit", "cm_lbvh_pairs_temp_bytes",
     "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times", "cm_set_stiffness", "cm_forward_ckpt",
-    "cm_adjoint", "cm_autotune",
+    "cm_adjoint", "cm_autotune", "cm_read_vertices",
 )
+# Rendering::VertexFormat as cm_read_vertices fills it (position, color, normal, uv)
+VERTEX_DTYPE = np.dtype([("position", "<f4", (3,)), ("color", "<f4", (4,)),
+                         ("normal", "<f4", (3,)), ("uv", "<f4", (2,))])

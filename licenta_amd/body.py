@@ -19,7 +19,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import CONTACT_DTYPE, check, lib, make_params, ptr
+from ._abi import CONTACT_DTYPE, VERTEX_DTYPE, check, lib, make_params, ptr
 
 BUF = dict(x=0, p=1, v=2, tri=3, keys=4, perm=5, lefts=6, rights=7, parents=8, rmll=9, rmlr=10,
            aabb_min=11, aabb_max=12, pairs=13, vf=14, ee=15)
@@ -143,6 +143,13 @@ class ClothBody:
         out = np.zeros((self.n, 3), dtype=np.float32)
         check(lib().cm_read_normals(self.h, ptr(out)))
         return out
+
+    def read_vertices(self, verts: np.ndarray) -> None:
+        """Fill the position and normal fields of a VERTEX_DTYPE array in place (the
+        reference's Update readback); color and uv are left as they are."""
+        if verts.dtype != VERTEX_DTYPE or verts.shape != (self.n,) or not verts.flags.c_contiguous:
+            raise ValueError("read_vertices needs a contiguous (n,) VERTEX_DTYPE array")
+        check(lib().cm_read_vertices(self.h, ptr(verts)))
 
     def set_state(self, x, v, p=None, tri=None) -> None:
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)

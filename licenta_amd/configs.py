@@ -33,6 +33,7 @@ F_SPHERE = 1 << 5
 F_EE_CONTACTS = 1 << 6
 F_FRICTION = 1 << 7
 F_BEND_FP64 = 1 << 8
+F_VERTEX_NORMALS = 1 << 9
 
 RC_STALE_LEAF_AABB = 1 << 0
 RC_BEND_ABS = 1 << 1
@@ -66,7 +67,7 @@ def default_params() -> dict:
         morton_normalize=0,
         pair_cap=64,
         flags=(F_SELF_COLLISION | F_GROUND | F_WORLD_CLAMP | F_DAMPING | F_LRA | F_EE_CONTACTS
-               | F_BEND_FP64),
+               | F_BEND_FP64 | F_VERTEX_NORMALS),
         ref_compat=RC_ALL,
     )
 

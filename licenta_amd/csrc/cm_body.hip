@@ -97,6 +97,11 @@ struct cm_body {
   uint8_t* lra_flag = nullptr;
   float4* ground = nullptr;
   float *damp_partials = nullptr, *damp_state = nullptr;
+  // vertex normals: incident triangles per vertex (CSR, ascending triangle index,
+  // vertex ids in the input order) and the normals
+  int* vt_prefix = nullptr;
+  int4* vt_tri = nullptr;
+  float4* vnrm = nullptr;
   // collision
   int4 *tri = nullptr, *tri_alt = nullptr;
   uint64_t* keys = nullptr;
@@ -453,6 +458,12 @@ int record_substep(cm_body* b, hipEvent_t* ev = nullptr, const TapeSlot* tape = 
   mark(CM_STAGE_PREDICT);
   int rc = launch_predict(pb, pp, s);
   if (rc) return rc;
+  // the carried vertex normals advance once per simulated substep: not again when
+  // the adjoint recomputes a segment
+  if ((P.flags & CM_F_VERTEX_NORMALS) && !tape) {
+    rc = launch_vertex_normals(b->n, b->vt_prefix, b->vt_tri, b->p, b->vnrm, 1, 1, s);
+    if (rc) return rc;
+  }
   const bool coll = (P.flags & CM_F_SELF_COLLISION) && b->t >= 2;
   mark(CM_STAGE_TRIANGLES);
   if (coll) {
@@ -584,7 +595,7 @@ void cm_default_params(cm_params* p) {
   p->morton_normalize = 0;
   p->pair_cap = 64;
   p->flags = CM_F_SELF_COLLISION | CM_F_GROUND | CM_F_WORLD_CLAMP | CM_F_DAMPING | CM_F_LRA |
-             CM_F_EE_CONTACTS | CM_F_BEND_FP64;
+             CM_F_EE_CONTACTS | CM_F_BEND_FP64 | CM_F_VERTEX_NORMALS;
   p->ref_compat = CM_RC_ALL;
 }
 
@@ -709,6 +720,31 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
   CM_HIP(hipMemset(b->warn, 0, 16));
   CM_HIP(hipMemset(b->counters, 0, 64));
   CM_HIP(hipMemset(b->ground, 0, sizeof(float4) * n));
+  {
+    std::vector<int> vt_prefix(n + 1, 0);
+    for (int64_t i = 0; i < 3 * (int64_t)t; ++i) ++vt_prefix[tri[i] + 1];
+    for (int i = 0; i < n; ++i) vt_prefix[i + 1] += vt_prefix[i];
+    std::vector<int> fill(vt_prefix.begin(), vt_prefix.end() - 1);
+    std::vector<int4> vt_tri(3 * (size_t)t);
+    for (int j = 0; j < t; ++j) {
+      const int4 q = make_int4((int)tri[3 * j], (int)tri[3 * j + 1], (int)tri[3 * j + 2], j);
+      for (int k = 0; k < 3; ++k) vt_tri[fill[tri[3 * j + k]]++] = q;
+    }
+    if ((rc = upload(b, &b->vt_prefix, vt_prefix)) || (rc = upload(b, &b->vt_tri, vt_tri)) ||
+        (rc = dalloc(b, &b->vnrm, n)))
+      return fail(rc);
+    CM_HIP(hipMemset(b->vnrm, 0, sizeof(float4) * n));
+    // the creation's normals: the raw (area-weighted) face normals of the input
+    // positions (Physics/CudaDeformableBody.cpp:214, 269)
+    if (params->flags & CM_F_VERTEX_NORMALS) {
+      rc = launch_vertex_normals(n, b->vt_prefix, b->vt_tri, b->x, b->vnrm, 0, 0, b->stream);
+      if (rc) return fail(rc);
+      if (hipStreamSynchronize(b->stream) != hipSuccess) {
+        set_error("cm_create: vertex normals failed");
+        return fail(CM_ERR_HIP);
+      }
+    }
+  }
   // collision buffers
   b->cap = std::min(params->pair_cap, t);
   const size_t m = 2 * (size_t)t - 1;
@@ -957,10 +993,9 @@ int cm_read_velocities(cm_body* b, float* xyz) {
   return read_f4(b, b->v, xyz);
 }
 
-// Area-independent vertex normals: normalised sum of the unit face normals of
-// the incident triangles, computed on the host from the carried positions
-// (Physics/CudaDeformableUtils.cu:190-249 sums face normals with float atomics;
-// here the sum runs in triangle order).
+// With CM_F_VERTEX_NORMALS the normals carried through the substeps (computed from
+// the predicted positions of each substep, as the reference's FixedUpdate does);
+// without it, the normals of the current positions, computed now on the device.
 int cm_read_normals(cm_body* b, float* out) {
   int rc = body_check(b);
   if (rc) return rc;
@@ -968,25 +1003,28 @@ int cm_read_normals(cm_body* b, float* out) {
     set_error("cm_read_normals: null output");
     return CM_ERR_ARG;
   }
-  std::vector<float> xyz(3 * (size_t)b->n);
-  rc = read_f4(b, b->x, xyz.data());
-  if (rc) return rc;
-  std::vector<int4> tri(b->t);
-  CM_HIP(hipMemcpy(tri.data(), b->tri, sizeof(int4) * b->t, hipMemcpyDeviceToHost));
-  std::vector<f3> acc(b->n, mk3(0, 0, 0));
-  for (int i = 0; i < b->t; ++i) {
-    const int4 q = tri[i];
-    auto P = [&](int k) { return mk3(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]); };
-    const f3 nrm = normalize(cross(sub(P(q.y), P(q.x)), sub(P(q.z), P(q.x))));
-    acc[q.x] = add(acc[q.x], nrm);
-    acc[q.y] = add(acc[q.y], nrm);
-    acc[q.z] = add(acc[q.z], nrm);
+  if (!(b->prm.flags & CM_F_VERTEX_NORMALS)) {
+    rc = launch_vertex_normals(b->n, b->vt_prefix, b->vt_tri, b->x, b->vnrm, 0, 1, b->stream);
+    if (rc) return rc;
   }
+  return read_f4(b, b->vnrm, out);
+}
+
+int cm_read_vertices(cm_body* b, float* verts) {
+  int rc = body_check(b);
+  if (rc) return rc;
+  if (!verts) {
+    set_error("cm_read_vertices: null output");
+    return CM_ERR_ARG;
+  }
+  std::vector<float> xyz(3 * (size_t)b->n), nrm(3 * (size_t)b->n);
+  if ((rc = cm_read_positions(b, xyz.data())) || (rc = cm_read_normals(b, nrm.data()))) return rc;
   for (int i = 0; i < b->n; ++i) {
-    const f3 nn = normalize(acc[i]);
-    out[3 * i] = nn.x;
-    out[3 * i + 1] = nn.y;
-    out[3 * i + 2] = nn.z;
+    float* r = verts + (size_t)CM_VERTEX_FLOATS * i;
+    for (int k = 0; k < 3; ++k) {
+      r[CM_VERTEX_POSITION + k] = xyz[3 * (size_t)i + k];
+      r[CM_VERTEX_NORMAL + k] = nrm[3 * (size_t)i + k];
+    }
   }
   return CM_OK;
 }

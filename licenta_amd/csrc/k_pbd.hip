@@ -17,6 +17,9 @@
 //                    reference's gathers (:1417-1475) without the global slot buffer.
 //  k_integrate    <- _FinalUpdate (:1916-1932) + _WorldConstraints
 //                    (Physics/CudaDeformableUtils.cu:260-269)
+//  k_vertex_normals <- DeformableUtils::UpdateVertexNormals
+//                    (Physics/CudaDeformableUtils.cu:190-249) with the unit face normals
+//                    of _UpdateTrianglesContinuous (:135)
 //  k_friction     <- CudaPBD::ApplyFriction (:770-833): _DampVelRigid (:1879-1912),
 //                    then _ApplyFriction / _AccumulateFriction / FinalFrictionStep
 //                    (:1725-1877, :1700-1723) over the per-vertex contact records
@@ -749,6 +752,32 @@ __global__ void __launch_bounds__(kBlock) k_friction(
   v[i] = f4(vel, vi.w);
 }
 
+// Vertex normals. The reference writes each triangle's face normal once per corner
+// and adds them into the per-vertex normals with float atomics, then normalises in
+// place (Physics/CudaDeformableUtils.cu:190-249); the per-vertex normals are never
+// cleared, so each substep adds to the previous substep's unit normal. The face
+// normals are unit ones in a substep (:135) and the raw cross products at creation
+// (:54, used by Physics/CudaDeformableBody.cpp:269). Here each vertex gathers its
+// incident triangles (vertex ids in their original order, listed by ascending
+// triangle index) and adds their face normals (unit = 1: normalised) to the carried
+// normal (carry = 1) or to zero (carry = 0) in that fixed order.
+__global__ void __launch_bounds__(kBlock) k_vertex_normals(int n, const int* __restrict__ vt_prefix,
+                                                           const int4* __restrict__ vt_tri,
+                                                           const float4* __restrict__ pos,
+                                                           float4* __restrict__ nrm, int carry,
+                                                           int unit) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  f3 acc = carry ? xyz(nrm[i]) : mk3(0.f, 0.f, 0.f);
+  for (int k = vt_prefix[i], e = vt_prefix[i + 1]; k < e; ++k) {
+    const int4 q = vt_tri[k];
+    const f3 a = xyz(pos[q.x]);
+    const f3 fn = cross(sub(xyz(pos[q.y]), a), sub(xyz(pos[q.z]), a));
+    acc = add(acc, unit ? normalize(fn) : fn);
+  }
+  nrm[i] = f4(normalize(acc), 0.f);
+}
+
 // ---- adjoint of the differentiable substep (no reference counterpart) ------------
 // The substep as a function of (x, v, kp): v' = v + dt g, p0 = x + dt v', I Jacobi
 // iterations p_{k+1} = p_k + (sum of the corrections at p_k) / count_k over the
@@ -1190,6 +1219,14 @@ int launch_integrate_from(const PbdBufs& b, const PbdParams& pp, const float4* p
   hipLaunchKernelGGL(k_integrate, dim3(nblocks(b.n)), dim3(kBlock), 0, s, b.n, pfin, b.x, b.v,
                      b.p, b.pinned, pp.dt, (pp.flags & CM_F_WORLD_CLAMP) ? 1 : 0, pp.wmin,
                      pp.wmax);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_vertex_normals(int n, const int* vt_prefix, const int4* vt_tri, const float4* pos,
+                          float4* nrm, int carry, int unit, hipStream_t s) {
+  hipLaunchKernelGGL(k_vertex_normals, dim3(nblocks(n)), dim3(kBlock), 0, s, n, vt_prefix, vt_tri,
+                     pos, nrm, carry, unit);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

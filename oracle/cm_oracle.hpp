@@ -893,6 +893,7 @@ struct Sim {
   bool record = false;
   std::vector<Tape> tape;
   std::vector<uint8_t> gflag;  // ground contacts of the last solver set-up
+  std::vector<RV> vnrm;        // vertex normals (carried with CM_F_VERTEX_NORMALS)
   bool sphere_on = false;
   F3 sphere_c{0, 0, 0};
   float sphere_r = 0.f;
@@ -927,6 +928,8 @@ struct Sim {
     rmll = lefts;
     rmlr = lefts;
     parents.assign(t > 0 ? 2 * (size_t)t - 1 : 0, 0);
+    vnrm.assign(n, mk<R>(0, 0, 0));
+    if (prm.flags & CM_F_VERTEX_NORMALS) vertex_normals(x, false, false);
     sphere_on = (prm.flags & CM_F_SPHERE) != 0;
     sphere_c = mk(prm.sphere_center[0], prm.sphere_center[1], prm.sphere_center[2]);
     sphere_r = prm.sphere_radius;
@@ -1452,6 +1455,27 @@ struct Sim {
     }
   }
 
+  // ---- vertex normals (Physics/CudaDeformableUtils.cu:190-249) ----
+  // Each triangle's face normal (CudaUtils::FaceNormal = cross(b-a, c-a)), a unit one
+  // in a substep (:135) and the raw cross product at creation (:54, used by
+  // Physics/CudaDeformableBody.cpp:269), is added to its three vertices' normals,
+  // which are then normalised. The reference adds with float atomics onto the
+  // previous substep's normal (never cleared); here the previous normal (carry) comes
+  // first, then the triangles in ascending input index.
+  void vertex_normals(const std::vector<RV>& pos, bool carry, bool unit = true) {
+    std::vector<RV> acc(n, mk<R>(0, 0, 0));
+    if (carry) acc = vnrm;
+    for (size_t j = 0; j < tp.tri.size(); ++j) {
+      const Tri& q = tp.tri[j];
+      const RV cr = cross(pos[q.v2] - pos[q.v1], pos[q.v3] - pos[q.v1]);
+      const RV fn = unit ? normalize(cr) : cr;
+      acc[q.v1] = acc[q.v1] + fn;
+      acc[q.v2] = acc[q.v2] + fn;
+      acc[q.v3] = acc[q.v3] + fn;
+    }
+    for (int i = 0; i < n; ++i) vnrm[i] = normalize(acc[i]);
+  }
+
   // ---- stage 8: friction on the integrated velocities (Physics/CudaPBD.cu:770-833,
   // called at Physics/CudaDeformableBody.cpp:596-602) ----
   void friction() {
@@ -1506,6 +1530,7 @@ struct Sim {
       vf_[i] = cvt<float>(v[i]);
     }
     predict<R>(x, v, p);
+    if (prm.flags & CM_F_VERTEX_NORMALS) vertex_normals(p, true);
     if (sizeof(R) == sizeof(float)) {
       for (int i = 0; i < n; ++i) {
         pf[i] = cvt<float>(p[i]);

@@ -3,6 +3,7 @@
 // Test infrastructure only: the product never links or calls it.
 #include <chrono>
 #include <memory>
+#include <type_traits>
 
 #include "cm_oracle.hpp"
 
@@ -118,6 +119,28 @@ int orc_get_state(void* hp, double* x, double* p, double* v) {
       if (p) { p[3 * i] = sim.p[i].x; p[3 * i + 1] = sim.p[i].y; p[3 * i + 2] = sim.p[i].z; }
       if (v) { v[3 * i] = sim.v[i].x; v[3 * i + 1] = sim.v[i].y; v[3 * i + 2] = sim.v[i].z; }
     }
+    return 0;
+  });
+}
+
+// vertex normals: with pos (3N) the normals of those positions, else the carried ones
+int orc_get_normals(void* hp, const double* pos, double* out) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) {
+    typedef typename std::decay<decltype(sim)>::type S;
+    const std::vector<typename S::RV> carried = sim.vnrm;
+    if (pos) {
+      std::vector<typename S::RV> q(sim.n);
+      for (int i = 0; i < sim.n; ++i)
+        q[i] = mk<decltype(q[0].x)>(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+      sim.vertex_normals(q, false);
+    }
+    for (int i = 0; i < sim.n; ++i) {
+      out[3 * i] = sim.vnrm[i].x;
+      out[3 * i + 1] = sim.vnrm[i].y;
+      out[3 * i + 2] = sim.vnrm[i].z;
+    }
+    sim.vnrm = carried;
     return 0;
   });
 }

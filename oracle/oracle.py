@@ -45,6 +45,7 @@ def lib():
     L.orc_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.orc_set_state.argtypes = [vp, vp, vp, vp, vp]
     L.orc_get_state.argtypes = [vp, vp, vp, vp]
+    L.orc_get_normals.argtypes = [vp, vp, vp]
     L.orc_set_parents.argtypes = [vp, vp]
     L.orc_counts.argtypes = [vp, vp]
     L.orc_warnings.argtypes = [vp]
@@ -161,6 +162,14 @@ class OracleSim:
         v = np.zeros((self.n, 3))
         lib().orc_get_state(self.h, _p(x), _p(p), _p(v))
         return x, p, v
+
+    def normals(self, pos=None) -> np.ndarray:
+        """Vertex normals: the carried ones (CM_F_VERTEX_NORMALS), or with pos (N, 3)
+        the normals of those positions (the carried ones are kept)."""
+        out = np.zeros((self.n, 3))
+        pos = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        lib().orc_get_normals(self.h, _p(pos), _p(out))
+        return out
 
     def get_i32(self, name: str) -> np.ndarray:
         n = lib().orc_get_i32(self.h, self.I32[name], None, 0)

@@ -69,3 +69,19 @@ def test_entry_points_reject_null_body():
     assert lib.cm_adjoint(None, _abi.ptr(buf), None, None, None, None, None) == -1
     assert lib.cm_autotune(None, 2, _abi.ptr(picks)) == -1
     assert b"null body" in lib.cm_last_error()
+
+
+def test_flag_and_layout_constants_match_header():
+    """configs.F_* / RC_* and the VertexFormat offsets agree with clothmi.h."""
+    from licenta_amd import configs
+    text = open(os.path.join(ROOT, "include", "clothmi.h")).read()
+    defs = dict(re.findall(r"#define (CM_(?:F|RC)_[A-Z0-9_]+) \(1u << (\d+)\)", text))
+    assert len(defs) >= 13
+    for name, bit in defs.items():
+        py = name[3:]
+        assert getattr(configs, py) == 1 << int(bit), name
+    assert configs.RC_ALL == sum(1 << int(b) for n, b in defs.items() if n.startswith("CM_RC_"))
+    lay = dict(re.findall(r"#define CM_VERTEX_([A-Z]+) (\d+)", text))
+    assert _abi.VERTEX_DTYPE.itemsize == 4 * int(lay["FLOATS"]) == 48
+    assert _abi.VERTEX_DTYPE.fields["position"][1] == 4 * int(lay["POSITION"])
+    assert _abi.VERTEX_DTYPE.fields["normal"][1] == 4 * int(lay["NORMAL"])

@@ -223,6 +223,61 @@ def test_friction_folded_on_ground(oracle_mod, tail):
     assert total > 0
 
 
+def test_carried_vertex_normals(oracle_mod):
+    """CM_F_VERTEX_NORMALS: the creation normals, then the normals carried through
+    four substeps of a wavy cloth (from each substep's predicted positions) against
+    the FP32 oracle run
+    alongside from the same start (not re-seeded, so the carried normals of both
+    build up over the whole run)."""
+    from licenta_amd.body import ClothBody
+    n = 24
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    xyz[:, :, 1] += 0.01 * np.sin(np.linspace(0, 3, n))[:, None]
+    sc["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    sc["params"] = dict(sc["params"], flags=sc["params"]["flags"] | configs.F_VERTEX_NORMALS)
+    body = ClothBody.from_scene(sc)
+    o = oracle_mod.OracleSim(sc, 32)
+    assert float(np.abs(body.normals() - o.normals()).max()) <= 1e-6  # creation normals
+    for s in range(4):
+        print(f"[normals] substep {s}", flush=True)
+        body.fixed_update(1)
+        o.step(1)
+        x32, _, _ = o.state()
+        assert rel_err(body.positions(), x32) <= 1e-5
+        err = float(np.abs(body.normals() - o.normals()).max())
+        print(f"  normal max abs err {err:.2e}", flush=True)
+        assert err <= 1e-5
+    body.close()
+
+
+def test_fresh_normals_and_vertex_format(oracle_mod):
+    """Without CM_F_VERTEX_NORMALS, cm_read_normals gives the normals of the current
+    positions; cm_read_vertices writes positions and normals into 48-byte
+    VertexFormat records and keeps their color and uv."""
+    from licenta_amd._abi import VERTEX_DTYPE
+    from licenta_amd.body import ClothBody
+    sc = configs.config_scene(0, 20)
+    sc["params"] = dict(sc["params"], flags=sc["params"]["flags"] & ~configs.F_VERTEX_NORMALS)
+    body = ClothBody.from_scene(sc)
+    body.fixed_update(5)
+    x = body.positions()
+    nrm = body.normals()
+    o = oracle_mod.OracleSim(sc, 32)
+    assert float(np.abs(nrm - o.normals(x)).max()) <= 1e-6
+    rng = np.random.default_rng(3)
+    verts = np.zeros(body.n, VERTEX_DTYPE)
+    verts["color"] = rng.random((body.n, 4), dtype=np.float32)
+    verts["uv"] = rng.random((body.n, 2), dtype=np.float32)
+    keep = verts.copy()
+    body.read_vertices(verts)
+    assert np.array_equal(verts["position"], x) and np.array_equal(verts["normal"], nrm)
+    assert np.array_equal(verts["color"], keep["color"]) and np.array_equal(verts["uv"], keep["uv"])
+    with pytest.raises(ValueError):
+        body.read_vertices(np.zeros(body.n - 1, VERTEX_DTYPE))
+    body.close()
+
+
 def test_graph_replay_matches_eager_state(oracle_mod):
     """Several substeps in one cm_step call equal the same substeps one by one."""
     from licenta_amd.body import ClothBody
