@@ -1583,13 +1583,14 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
   CM_HIP(hipStreamSynchronize(b->stream));
   drop_graph(b);
   // every trial starts from the current state
-  float4 *sx = nullptr, *sv = nullptr, *sp = nullptr;
+  float4 *sx = nullptr, *sv = nullptr, *sp = nullptr, *snrm = nullptr;
   int4* stri = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto cleanup = [&]() {
     hipFree(sx);
     hipFree(sv);
     hipFree(sp);
+    hipFree(snrm);
     hipFree(stri);
     if (e0) hipEventDestroy(e0);
     if (e1) hipEventDestroy(e1);
@@ -1597,6 +1598,7 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
   if (hipMalloc(&sx, sizeof(float4) * n) != hipSuccess ||
       hipMalloc(&sv, sizeof(float4) * n) != hipSuccess ||
       hipMalloc(&sp, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&snrm, sizeof(float4) * n) != hipSuccess ||
       hipMalloc(&stri, sizeof(int4) * t) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess) {
     cleanup();
@@ -1613,6 +1615,7 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
     cp(sx, b->x, sizeof(float4) * n);
     cp(sv, b->v, sizeof(float4) * n);
     cp(sp, b->p, sizeof(float4) * n);
+    cp(snrm, b->vnrm, sizeof(float4) * n);  // carried vertex normals
     cp(stri, b->tri, sizeof(int4) * t);
     return e;
   };

@@ -110,17 +110,19 @@ def test_batch_rejects_unsupported_modes():
 
 def test_autotune_keeps_results():
     """cm_autotune only changes launch shapes: after tuning, a batch steps to the
-    same positions, pairs and contacts, bit for bit, as an untuned one, and tuning
-    leaves the state where it was."""
+    same positions, pairs, contacts and carried vertex normals, bit for bit, as an
+    untuned one, and tuning leaves the state where it was."""
     from licenta_amd.body import ClothBody
     print("autotune on a 12-cloth batch", flush=True)
     scenes = configs.batched_scene(2, count=12)
-    a, b = ClothBody.batch(scenes), ClothBody.batch(scenes)
+    prm = dict(scenes[0]["params"], flags=scenes[0]["params"]["flags"] | configs.F_VERTEX_NORMALS)
+    a, b = ClothBody.batch(scenes, prm), ClothBody.batch(scenes, prm)
     a.fixed_update(3)
     b.fixed_update(3)
-    x_before = b.positions()
+    x_before, n_before = b.positions(), b.normals()
     picks = b.autotune(2)
     assert np.array_equal(b.positions(), x_before)
+    assert np.array_equal(b.normals(), n_before)
     assert picks["tile_vertices"] in (64, 128, 256)
     assert picks["traversal_chunk"] in (1, 2, 4, 8)
     assert picks["secant_workgroups"] in (1024, 2048, 4096)
@@ -128,6 +130,7 @@ def test_autotune_keeps_results():
     b.fixed_update(4)
     assert np.array_equal(a.positions(), b.positions())
     assert np.array_equal(a.velocities(), b.velocities())
+    assert np.array_equal(a.normals(), b.normals())
     assert np.array_equal(a.pairs(), b.pairs())
     for ca, cb in zip(a.contacts(), b.contacts()):
         assert np.array_equal(ca, cb)
