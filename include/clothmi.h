@@ -90,6 +90,12 @@ extern "C" {
  * substep's normal, then normalised. Without it cm_read_normals computes the
  * normals of the current positions when called. */
 #define CM_F_VERTEX_NORMALS (1u << 9)
+/* Time-of-contact rollback before the solver: each vertex in contacts moves to
+ * (1 - t) x + t p with t the smallest time of its contacts (CudaPBD::RevertToTOC,
+ * Physics/CudaPBD.cu:727-768). The reference's call is commented out
+ * (Physics/CudaDeformableBody.cpp:583), so it is off by default. Not with
+ * cm_forward_ckpt. */
+#define CM_F_REVERT_TOC (1u << 10)
 
 /* ---- reference-compatibility bits (cm_params.ref_compat) ----------------- */
 /* Leaf AABBs stay in pre-sort triangle slots after the Morton sort
@@ -263,7 +269,7 @@ int cm_stage_times(cm_body* body, int32_t substeps, float* ms);
  * the substep's VF / EE contacts (frozen: contact sets and Jacobi counts do not
  * carry gradient), and the velocity update; the Morton / sort / LBVH / CCD stages
  * carry no gradient. Bodies with ground, long-range attachments, a sphere,
- * damping or friction are rejected (CM_ERR_ARG). */
+ * damping, friction or contact-time rollback are rejected (CM_ERR_ARG). */
 
 /* Replace the stiffness of every distance (k_dist, S floats) and bend (k_bend,
  * B floats) constraint, in [0, 1]; either may be NULL (kept). Per-iteration

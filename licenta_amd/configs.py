@@ -34,6 +34,7 @@ F_EE_CONTACTS = 1 << 6
 F_FRICTION = 1 << 7
 F_BEND_FP64 = 1 << 8
 F_VERTEX_NORMALS = 1 << 9
+F_REVERT_TOC = 1 << 10
 
 RC_STALE_LEAF_AABB = 1 << 0
 RC_BEND_ABS = 1 << 1

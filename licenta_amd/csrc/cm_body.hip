@@ -505,6 +505,10 @@ int record_substep(cm_body* b, hipEvent_t* ev = nullptr, const TapeSlot* tape = 
     if (rc) return rc;
     rc = launch_dynamic_csr(b->nb, s);
     if (rc) return rc;
+    if (P.flags & CM_F_REVERT_TOC) {
+      rc = launch_revert_toc(pb, b->nb, s);
+      if (rc) return rc;
+    }
     rc = launch_copy_i4(b->t, b->tri_alt, b->tri, s);
     if (rc) return rc;
     if (tape) {
@@ -1282,10 +1286,11 @@ void adj_release(cm_body* b) {
 }
 
 int adj_check(cm_body* b, const char* who) {
-  const uint32_t off = CM_F_GROUND | CM_F_LRA | CM_F_SPHERE | CM_F_DAMPING | CM_F_FRICTION;
+  const uint32_t off =
+      CM_F_GROUND | CM_F_LRA | CM_F_SPHERE | CM_F_DAMPING | CM_F_FRICTION | CM_F_REVERT_TOC;
   if (b->prm.flags & off) {
     set_error(std::string(who) + ": the differentiable substep has no ground, long-range "
-                                 "attachments, sphere, damping or friction");
+                                 "attachments, sphere, damping, friction or contact-time rollback");
     return CM_ERR_ARG;
   }
   if (b->ntiles <= 0) {

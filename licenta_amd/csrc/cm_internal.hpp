@@ -214,6 +214,7 @@ int launch_integrate(const PbdBufs& b, const PbdParams& pp, hipStream_t s);
 int launch_integrate_from(const PbdBufs& b, const PbdParams& pp, const float4* pfin,
                           hipStream_t s);
 int launch_friction(const PbdBufs& b, const PbdParams& pp, const NarrowBufs& nb, hipStream_t s);
+int launch_revert_toc(const PbdBufs& b, const NarrowBufs& nb, hipStream_t s);
 int launch_vertex_normals(int n, const int* vt_prefix, const int4* vt_tri, const float4* pos,
                           float4* nrm, int carry, int unit, hipStream_t s);
 // the solver stage with every iterate kept: pk[k * n, (k + 1) * n) = positions

@@ -20,6 +20,8 @@
 //  k_vertex_normals <- DeformableUtils::UpdateVertexNormals
 //                    (Physics/CudaDeformableUtils.cu:190-249) with the unit face normals
 //                    of _UpdateTrianglesContinuous (:135)
+//  k_revert_toc   <- CudaPBD::RevertToTOC (:727-768, FillTOC :2014-2046, _RevertToTOC
+//                    :2048-2072) over the per-vertex contact records
 //  k_friction     <- CudaPBD::ApplyFriction (:770-833): _DampVelRigid (:1879-1912),
 //                    then _ApplyFriction / _AccumulateFriction / FinalFrictionStep
 //                    (:1725-1877, :1700-1723) over the per-vertex contact records
@@ -752,6 +754,33 @@ __global__ void __launch_bounds__(kBlock) k_friction(
   v[i] = f4(vel, vi.w);
 }
 
+// Time-of-contact rollback between the narrow phase and the solver (the reference's
+// call at Physics/CudaDeformableBody.cpp:583 is commented out; CM_F_REVERT_TOC turns
+// it on). A vertex in contacts takes the smallest contact time t of its contacts
+// (the reference starts its minimum at 100) and moves to (1 - t) x + t p
+// (CudaUtils::AdvancePositionInTimePos, Core/CudaUtils.cu:396-399). The contact t is
+// the time of contact in seconds, used as the blend weight as the reference does.
+__global__ void __launch_bounds__(kBlock) k_revert_toc(int n, const int* __restrict__ dprefix,
+                                                       const uint32_t* __restrict__ drec,
+                                                       const int* __restrict__ d_nvf,
+                                                       const float4* __restrict__ vf_tn,
+                                                       const float4* __restrict__ ee_tn,
+                                                       const float4* __restrict__ x,
+                                                       float4* __restrict__ p) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int b = dprefix[i], e = dprefix[i + 1];
+  if (e == b) return;
+  const int nvf = *d_nvf;
+  float tmin = 100.f;
+  for (int k = b; k < e; ++k) {
+    const int c = (int)(drec[k] >> 2);
+    tmin = fminf(tmin, (c < nvf ? vf_tn[c] : ee_tn[c - nvf]).x);
+  }
+  const float4 pi = p[i];
+  p[i] = f4(add(mul(xyz(x[i]), 1.f - tmin), mul(xyz(pi), tmin)), pi.w);
+}
+
 // Vertex normals. The reference writes each triangle's face normal once per corner
 // and adds them into the per-vertex normals with float atomics, then normalises in
 // place (Physics/CudaDeformableUtils.cu:190-249); the per-vertex normals are never
@@ -1227,6 +1256,13 @@ int launch_vertex_normals(int n, const int* vt_prefix, const int4* vt_tri, const
                           float4* nrm, int carry, int unit, hipStream_t s) {
   hipLaunchKernelGGL(k_vertex_normals, dim3(nblocks(n)), dim3(kBlock), 0, s, n, vt_prefix, vt_tri,
                      pos, nrm, carry, unit);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
+int launch_revert_toc(const PbdBufs& b, const NarrowBufs& nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_revert_toc, dim3(nblocks(b.n)), dim3(kBlock), 0, s, b.n, nb.dprefix,
+                     nb.drec, nb.d_nvf, nb.vf_tn, nb.ee_tn, b.x, b.p);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

@@ -1455,6 +1455,24 @@ struct Sim {
     }
   }
 
+  // ---- time-of-contact rollback (Physics/CudaPBD.cu:727-768, FillTOC :2014-2046,
+  // _RevertToTOC :2048-2072; its call at Physics/CudaDeformableBody.cpp:583 is
+  // commented out) ----
+  // A vertex in contacts takes the smallest time t of its contacts (minimum started
+  // at 100) and moves to (1 - t) x + t p (Core/CudaUtils.cu:396-399).
+  void revert_toc() {
+    std::vector<float> tmin(n, 100.f);
+    std::vector<uint8_t> hit(n, 0);
+    for (const std::vector<cm_contact>* cs : {&vf, &ee})
+      for (const cm_contact& c : *cs)
+        for (int k = 0; k < 4; ++k) {
+          tmin[c.v[k]] = std::fmin(tmin[c.v[k]], c.t);
+          hit[c.v[k]] = 1;
+        }
+    for (int i = 0; i < n; ++i)
+      if (hit[i]) p[i] = mul(x[i], R(1) - R(tmin[i])) + mul(p[i], R(tmin[i]));
+  }
+
   // ---- vertex normals (Physics/CudaDeformableUtils.cu:190-249) ----
   // Each triangle's face normal (CudaUtils::FaceNormal = cross(b-a, c-a)), a unit one
   // in a substep (:135) and the raw cross product at creation (:54, used by
@@ -1544,6 +1562,7 @@ struct Sim {
       tape.back().vf = vf;
       tape.back().ee = ee;
     }
+    if (prm.flags & CM_F_REVERT_TOC) revert_toc();
     solve();
     integrate();
     if (prm.flags & CM_F_FRICTION) friction();

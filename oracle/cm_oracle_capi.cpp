@@ -123,6 +123,22 @@ int orc_get_state(void* hp, double* x, double* p, double* v) {
   });
 }
 
+// positions at the start of Jacobi iteration k (k = iterations: after the last) of
+// recorded substep s; -1 when not recorded
+int orc_tape_positions(void* hp, int s, int k, double* out) {
+  Handle* h = (Handle*)hp;
+  return with<float>(h, [&](auto& sim) {
+    if (s < 0 || s >= (int)sim.tape.size() || k < 0 || k >= (int)sim.tape[s].pk.size()) return -1;
+    const auto& q = sim.tape[s].pk[k];
+    for (int i = 0; i < sim.n; ++i) {
+      out[3 * i] = q[i].x;
+      out[3 * i + 1] = q[i].y;
+      out[3 * i + 2] = q[i].z;
+    }
+    return 0;
+  });
+}
+
 // vertex normals: with pos (3N) the normals of those positions, else the carried ones
 int orc_get_normals(void* hp, const double* pos, double* out) {
   Handle* h = (Handle*)hp;

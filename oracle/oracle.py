@@ -46,6 +46,7 @@ def lib():
     L.orc_set_state.argtypes = [vp, vp, vp, vp, vp]
     L.orc_get_state.argtypes = [vp, vp, vp, vp]
     L.orc_get_normals.argtypes = [vp, vp, vp]
+    L.orc_tape_positions.argtypes = [vp, i32, i32, vp]
     L.orc_set_parents.argtypes = [vp, vp]
     L.orc_counts.argtypes = [vp, vp]
     L.orc_warnings.argtypes = [vp]
@@ -162,6 +163,14 @@ class OracleSim:
         v = np.zeros((self.n, 3))
         lib().orc_get_state(self.h, _p(x), _p(p), _p(v))
         return x, p, v
+
+    def tape_positions(self, substep: int, iteration: int) -> np.ndarray:
+        """Positions at the start of Jacobi iteration `iteration` of recorded substep
+        `substep` (record() must be on)."""
+        out = np.zeros((self.n, 3))
+        if lib().orc_tape_positions(self.h, substep, iteration, _p(out)) != 0:
+            raise IndexError("not recorded")
+        return out
 
     def normals(self, pos=None) -> np.ndarray:
         """Vertex normals: the carried ones (CM_F_VERTEX_NORMALS), or with pos (N, 3)

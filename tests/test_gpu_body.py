@@ -223,6 +223,42 @@ def test_friction_folded_on_ground(oracle_mod, tail):
     assert total > 0
 
 
+def test_revert_to_toc(oracle_mod):
+    """CM_F_REVERT_TOC: two triangles meeting inside a substep at three contact times
+    (the oracle's closed-form case), then a folded cloth whose contacts start in
+    contact (t = 0, so contact vertices return to their start)."""
+    from licenta_amd.body import ClothBody
+    prm = configs.default_params()
+    prm.update(gravity=(0.0, 0.0, 0.0), k_stretch=0.5, k_bend=0.0, thickness=0.001,
+               solver_iterations=2, morton_bits=30, morton_normalize=1,
+               flags=configs.F_SELF_COLLISION | configs.F_EE_CONTACTS | configs.F_REVERT_TOC)
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1], [0.2, 0.00101, 0.2], [0.3, 0.001012, 0.2],
+                    [0.2, 0.001015, 0.3]], np.float32)
+    sc = dict(name="toc_pair", params=prm, xyz=xyz, tri=np.array([[0, 2, 1], [3, 5, 4]], np.uint32),
+              inv_mass=np.ones(6, np.float32), pinned=None, dist_pairs=None, k_dist=None,
+              k_bend=None)
+    body = ClothBody.from_scene(sc)
+    v = np.zeros((6, 3), np.float32)
+    v[:3, 1], v[3:, 1] = 0.3, -0.3
+    body.set_state(xyz, v)
+    print("[toc_pair] substep 0", flush=True)
+    r = compare_substep(oracle_mod, sc, body)
+    assert r["vf"] == 3
+    body.close()
+    n = 24
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    h = n // 2
+    for i in range(h, n):
+        xyz[i, :, 0] = xyz[2 * h - 1 - i, :, 0] + 0.002
+        xyz[i, :, 1] = 0.003 + 0.0005 * (i - h)
+    sc["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    sc["params"] = dict(sc["params"], thickness=0.004,
+                        flags=sc["params"]["flags"] | configs.F_REVERT_TOC)
+    res, w = run_scene(oracle_mod, sc, 3)
+    assert sum(r["vf"] + r["ee"] for r in res) > 0
+
+
 def test_carried_vertex_normals(oracle_mod):
     """CM_F_VERTEX_NORMALS: the creation normals, then the normals carried through
     four substeps of a wavy cloth (from each substep's predicted positions) against

@@ -310,3 +310,40 @@ def test_friction_contacts_invariants(oracle_mod, prec):
     np.testing.assert_allclose(dn_b, -((1.0 - k32) ** cnt[touched]) * dn_a, rtol=0, atol=tol * scale)
     np.testing.assert_allclose(vb[touched] - dn_b[:, None] * f, va[touched] - dn_a[:, None] * f,
                                rtol=0, atol=tol * scale)
+
+
+def test_revert_to_toc_closed_form(oracle_mod):
+    """CM_F_REVERT_TOC: a small tilted triangle and a large one move towards each other
+    and touch within the substep at three different contact times. The positions the solver
+    starts from are the predicted ones, except that a vertex in contacts sits at
+    (1 - t) x + t p with t the smallest time of its contacts (from the contact lists;
+    each vertex of the large triangle is in all three contacts)."""
+    prm = configs.default_params()
+    prm.update(gravity=(0.0, 0.0, 0.0), k_stretch=0.5, k_bend=0.0, thickness=0.001,
+               solver_iterations=2, morton_bits=30, morton_normalize=1,
+               flags=configs.F_SELF_COLLISION | configs.F_EE_CONTACTS | configs.F_REVERT_TOC)
+    gaps = (0.00101, 0.001012, 0.001015)  # just above the thickness
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1], [0.2, gaps[0], 0.2], [0.3, gaps[1], 0.2],
+                    [0.2, gaps[2], 0.3]], np.float32)
+    sc = _scene(xyz, [[0, 2, 1], [3, 5, 4]], np.ones(6), params=prm)
+    o = oracle_mod.OracleSim(sc, 64)
+    v0 = np.zeros((6, 3))
+    v0[:3, 1] = 0.3  # both move, so every vertex's blend depends on its t
+    v0[3:, 1] = -0.3
+    x0 = xyz.astype(np.float64)
+    o.set_state(x0, v0)
+    o.record(True)
+    o.step(1)
+    p0 = o.tape_positions(0, 0)
+    dt = np.float64(np.float32(prm["timestep"]))
+    pred = x0 + dt * v0
+    cs = np.concatenate([o.contacts(False), o.contacts(True)])
+    assert len(cs) == 3 and len(set(cs["t"].tolist())) == 3 and (cs["t"] > 0).all()
+    tmin = np.full(6, np.inf)
+    for c in cs:
+        for k in range(4):
+            tmin[c["v"][k]] = min(tmin[c["v"][k]], np.float64(c["t"]))
+    assert np.isfinite(tmin).all()
+    want = (1 - tmin[:, None]) * x0 + tmin[:, None] * pred
+    np.testing.assert_allclose(p0, want, rtol=0, atol=1e-15)
+    assert not np.allclose(p0, pred, rtol=0, atol=1e-9)
