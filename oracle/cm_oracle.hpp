@@ -25,7 +25,12 @@
 //    the critical point (reference reads an uninitialised value,
 //    Physics/CudaCubicSolver.cu:31-34, 110-114);
 //  * vector length: sqrt(x*x+y*y+z*z) in place of CUDA's norm3df
-//    (Core/CudaUtils.cu:212-220), parity unpinned at the last ulp.
+//    (Core/CudaUtils.cu:212-220), parity unpinned at the last ulp;
+//  * vertex normals: the per-vertex sum runs over the carried normal, then the
+//    incident triangles by ascending index (reference: float atomics,
+//    Physics/CudaDeformableUtils.cu:230-240);
+//  * friction: an EE contact's entries carry its own four vertex ids (reference:
+//    left unwritten, Physics/CudaPBD.cu:1810-1813).
 #pragma once
 
 #include <algorithm>
