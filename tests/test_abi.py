@@ -69,6 +69,30 @@ def test_entry_points_reject_null_body():
     assert lib.cm_adjoint(None, _abi.ptr(buf), None, None, None, None, None) == -1
     assert lib.cm_autotune(None, 2, _abi.ptr(picks)) == -1
     assert b"null body" in lib.cm_last_error()
+    assert lib.cm_read_vertices(None, _abi.ptr(buf)) == -1
+
+
+def test_batch_rejects_per_body_stages_before_any_device_call():
+    """cm_create_batch refuses LRA, rigid-mode damping and friction from the
+    parameters alone (no device needed), with a message naming them."""
+    from licenta_amd import configs
+    lib = _abi.lib()
+    xyz = np.zeros((3, 3), np.float32)
+    xyz[1, 0] = xyz[2, 2] = 1.0
+    tri = np.array([0, 1, 2], np.uint32)
+    inv = np.ones(3, np.float32)
+    n = np.array([3], np.int32)
+    t = np.array([1], np.int32)
+    for flag in (configs.F_LRA, configs.F_DAMPING, configs.F_FRICTION):
+        prm = configs.default_params()
+        prm.update(morton_bits=30, morton_normalize=1,
+                   flags=configs.F_SELF_COLLISION | flag)
+        p = _abi.make_params(prm)
+        h = C.c_void_p()
+        rc = lib.cm_create_batch(C.byref(p), 1, _abi.ptr(n), _abi.ptr(t), None, _abi.ptr(xyz),
+                                 _abi.ptr(tri), None, _abi.ptr(inv), None, None, None, C.byref(h))
+        assert rc == -1 and h.value is None
+        assert b"friction" in lib.cm_last_error()
 
 
 def test_flag_and_layout_constants_match_header():
