@@ -2,7 +2,8 @@
 free fall, pinned vertices, a two-constraint stretch chain with Jacobi
 averaging, the dihedral-bend correction direction against a finite-difference
 gradient of the cosine, the analytic sphere hit, the ground projection with its
-count, and the topology counts of a grid."""
+count, the topology counts of a grid, the friction's ground reflection and its
+contact invariants, and the time-of-contact rollback."""
 import numpy as np
 import pytest
 
