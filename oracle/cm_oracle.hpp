@@ -886,6 +886,9 @@ struct Sim {
   std::vector<int32_t> lefts, rights, parents, rmll, rmlr;
   std::vector<int32_t> pairs;  // 2 * P
   std::vector<cm_contact> vf, ee;
+  // CCD inputs of the last substep: the VF / EE candidates that passed the
+  // ownership gate and the feature swept-AABB filter, in candidate order
+  std::vector<cm_contact> cand_vf, cand_ee;
   uint32_t warn = 0;
   // differentiable rollout (no reference counterpart): with `record` set, step()
   // keeps each substep's entering state, the positions at the start of every Jacobi
@@ -897,6 +900,13 @@ struct Sim {
   };
   bool record = false;
   std::vector<Tape> tape;
+  // Contact activation decisions of the last solver stage, one byte per (iteration,
+  // contact) with the VF contacts first: act_mode 1 records them, 2 replays the
+  // ones given (a run in another precision on the same contact set); act_mismatch
+  // is set when the given decisions do not fit the contact set.
+  int act_mode = 0;
+  std::vector<uint8_t> act;
+  bool act_mismatch = false;
   std::vector<uint8_t> gflag;  // ground contacts of the last solver set-up
   std::vector<RV> vnrm;        // vertex normals (carried with CM_F_VERTEX_NORMALS)
   bool sphere_on = false;
@@ -1097,7 +1107,10 @@ struct Sim {
           }
         }
     }
-    std::vector<cm_contact> vfc, eec;
+    std::vector<cm_contact>& vfc = cand_vf;
+    std::vector<cm_contact>& eec = cand_ee;
+    vfc.clear();
+    eec.clear();
     for (int id = 0; id < nth; ++id) {
       vfc.insert(vfc.end(), pvf[id].begin(), pvf[id].end());
       eec.insert(eec.end(), pee[id].begin(), pee[id].end());
@@ -1116,6 +1129,8 @@ struct Sim {
   void vf_clear() {
     vf.clear();
     ee.clear();
+    cand_vf.clear();
+    cand_ee.clear();
   }
 
   void compute_keys(const std::vector<F3>& cen) {
@@ -1220,46 +1235,124 @@ struct Sim {
     c[2] = mul(q3, -(w3 * ts));
     c[3] = mul(q4, -(w4 * ts));
   }
-  void contact_vf(const cm_contact& k, const std::vector<RV>& ps, RV c[4]) const {
+  // Contact projections (Physics/CudaPBD.cu:1212-1311 VF, :1313-1415 EE) on the
+  // contact's four positions p and inverse masses w. act < 0: the constraint is
+  // active when its value cc = |n| - thickness is negative (`s = c < 0 ? s : 0`);
+  // act = 0 / 1 forces the decision (replay of another precision's decisions).
+  // Returns whether it was active.
+  static bool contact_vf_core(const cm_contact& k, const RV p4[4], const R w4[4], R th, RV c[4],
+                              int act = -1) {
     R b0 = fclamp(R(-k.w[1]), R(0), R(1)), b1 = fclamp(R(-k.w[2]), R(0), R(1)),
       b2 = fclamp(R(-k.w[3]), R(0), R(1));
-    RV pp = ps[k.v[0]], p0 = ps[k.v[1]], p1 = ps[k.v[2]], p2 = ps[k.v[3]];
-    R ww = w[k.v[0]], w0 = w[k.v[1]], w1 = w[k.v[2]], w2 = w[k.v[3]];
+    RV pp = p4[0], p0 = p4[1], p1 = p4[2], p2 = p4[3];
+    R ww = w4[0], w0 = w4[1], w1 = w4[2], w2 = w4[3];
     RV q = mul(p0, b0) + mul(p1, b1) + mul(p2, b2);
     RV nn = pp - q;
     R l = len(nn);
     nn = divs(nn, l);
-    R cc = l - R(prm.thickness);
+    R cc = l - th;
     RV g = nn, g0 = mul(-nn, b0), g1 = mul(-nn, b1), g2 = mul(-nn, b2);
     R s = ww + w0 * b0 * b0 + w1 * b1 * b1 + w2 * b2 * b2;
     for (int r = 0; r < 4; ++r) c[r] = mk<R>(0, 0, 0);
-    if (s == 0) return;
+    const bool on = act < 0 ? cc < 0 : act > 0;
+    if (s == 0) return on;
     s = cc / s;
-    s = cc < 0 ? s : R(0);
+    s = on ? s : R(0);
     c[0] = mul(g, -s * ww);
     c[1] = mul(g0, -s * w0);
     c[2] = mul(g1, -s * w1);
     c[3] = mul(g2, -s * w2);
+    return on;
   }
-  void contact_ee(const cm_contact& k, const std::vector<RV>& ps, RV c[4]) const {
+  static bool contact_ee_core(const cm_contact& k, const RV p4[4], const R w4[4], R th, RV c[4],
+                              int act = -1) {
     R b0 = R(k.w[0]), b1 = R(k.w[1]), b2 = R(-k.w[2]), b3 = R(-k.w[3]);
-    RV p0 = ps[k.v[0]], p1 = ps[k.v[1]], p2 = ps[k.v[2]], p3 = ps[k.v[3]];
-    R w0 = w[k.v[0]], w1 = w[k.v[1]], w2 = w[k.v[2]], w3 = w[k.v[3]];
+    RV p0 = p4[0], p1 = p4[1], p2 = p4[2], p3 = p4[3];
+    R w0 = w4[0], w1 = w4[1], w2 = w4[2], w3 = w4[3];
     RV q0 = mul(p0, b0) + mul(p1, b1), q1 = mul(p2, b2) + mul(p3, b3);
     RV nn = q0 - q1;
     R l = len(nn);
     nn = divs(nn, l);
-    R cc = l - R(prm.thickness);
+    R cc = l - th;
     RV g0 = mul(nn, b0), g1 = mul(nn, b1), g2 = mul(-nn, b2), g3 = mul(-nn, b3);
     R s = w0 * b0 * b0 + w1 * b1 * b1 + w2 * b2 * b2 + w3 * b3 * b3;
     for (int r = 0; r < 4; ++r) c[r] = mk<R>(0, 0, 0);
-    if (s == 0) return;
+    const bool on = act < 0 ? cc < 0 : act > 0;
+    if (s == 0) return on;
     s = cc / s;
-    s = cc < 0 ? s : R(0);
+    s = on ? s : R(0);
     c[0] = mul(g0, -s * w0);
     c[1] = mul(g1, -s * w1);
     c[2] = mul(g2, -s * w2);
     c[3] = mul(g3, -s * w3);
+    return on;
+  }
+  bool contact_vf(const cm_contact& k, const std::vector<RV>& ps, RV c[4], int act = -1) const {
+    RV p4[4];
+    R w4[4];
+    for (int r = 0; r < 4; ++r) {
+      p4[r] = ps[k.v[r]];
+      w4[r] = w[k.v[r]];
+    }
+    return contact_vf_core(k, p4, w4, R(prm.thickness), c, act);
+  }
+  bool contact_ee(const cm_contact& k, const std::vector<RV>& ps, RV c[4], int act = -1) const {
+    RV p4[4];
+    R w4[4];
+    for (int r = 0; r < 4; ++r) {
+      p4[r] = ps[k.v[r]];
+      w4[r] = w[k.v[r]];
+    }
+    return contact_ee_core(k, p4, w4, R(prm.thickness), c, act);
+  }
+
+  // Per-vertex sum of the dynamic corrections (Physics/CudaPBD.cu:711-720 sort by
+  // vertex + run-length encoding, then _ApplyDynamicCorrections :1448-1475): the
+  // records (vertex rec_id[k], value rec[k]) are visited per vertex in record order;
+  // only non-zero values are added and counted.
+  static void accumulate_records(const std::vector<uint32_t>& rec_id,
+                                 const std::vector<uint32_t>& order, const std::vector<RV>& rec,
+                                 std::vector<RV>& acc, std::vector<int>& cnt) {
+    size_t k = 0;
+    while (k < order.size()) {
+      uint32_t vid = rec_id[order[k]];
+      RV a = mk<R>(0, 0, 0);
+      int c = 0;
+      size_t e = k;
+      while (e < order.size() && rec_id[order[e]] == vid) {
+        const RV& val = rec[order[e]];
+        if (!is_zero(val)) {
+          a = a + val;
+          ++c;
+        }
+        ++e;
+      }
+      acc[vid] = acc[vid] + a;
+      cnt[vid] += c;
+      k = e;
+    }
+  }
+  static void record_order(const std::vector<uint32_t>& rec_id, std::vector<uint32_t>& order) {
+    order.resize(rec_id.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return rec_id[a] < rec_id[b]; });
+  }
+
+  // Long-range attachments of one vertex (Physics/CudaPBD.cu:1656-1688): for every
+  // flagged pinned vertex j, -(|p - p_j| - l0_j) along (p - p_j)/|p - p_j|, one count each.
+  static void lra_vertex(RV pi, int F, const int32_t* fixed_ids, const float* l0,
+                         const uint8_t* flags, const std::vector<RV>& ps, RV& a, int& c) {
+    a = mk<R>(0, 0, 0);
+    c = 0;
+    for (int j = 0; j < F; ++j)
+      if (flags[j]) {
+        RV diff = pi - ps[fixed_ids[j]];
+        R l = len(diff);
+        RV nn = divs(diff, l);
+        a = a + mul(nn, -(l - R(l0[j])));
+        ++c;
+      }
   }
 
   // Core/.../CudaDeformableUtils RaySphere (Collision/CudaCollisionDetection.cu:529-567)
@@ -1341,9 +1434,12 @@ struct Sim {
       const cm_contact& k = c < vf.size() ? vf[c] : ee[c - vf.size()];
       for (int r = 0; r < 4; ++r) rec_id[4 * c + r] = (uint32_t)k.v[r];
     }
-    for (size_t i = 0; i < order.size(); ++i) order[i] = (uint32_t)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return rec_id[a] < rec_id[b]; });
+    record_order(rec_id, order);
+    if (act_mode == 1) act.assign((size_t)prm.solver_iterations * C, 0);
+    if (act_mode == 2 && act.size() != (size_t)prm.solver_iterations * C) {
+      act_mismatch = true;  // decisions of another contact set: evaluate them here
+      act_mode = 0;
+    }
     const int S = (int)tp.sid1.size(), B = (int)tp.bkp.size();
     std::vector<RV> slots(tp.prefix[n]);
     std::vector<RV> acc(n);
@@ -1366,16 +1462,10 @@ struct Sim {
       // LRA corrections (Physics/CudaPBD.cu:1656-1688)
       if (!lflag.empty())
         for (int i = 0; i < n; ++i) {
-          RV a = mk<R>(0, 0, 0);
-          int c = 0;
-          for (int j = 0; j < F; ++j)
-            if (lflag[(size_t)i * F + j]) {
-              RV diff = p[i] - p[tp.lra_fixed[j]];
-              R l = len(diff);
-              RV nn = divs(diff, l);
-              a = a + mul(nn, -(l - R(tp.lra_l0[(size_t)i * F + j])));
-              ++c;
-            }
+          RV a;
+          int c;
+          lra_vertex(p[i], F, tp.lra_fixed.data(), tp.lra_l0.data() + (size_t)i * F,
+                     lflag.data() + (size_t)i * F, p, a, c);
           acc[i] = acc[i] + a;
           cnt[i] += c;
         }
@@ -1402,33 +1492,20 @@ struct Sim {
       }
       // dynamic constraints (:1190-1415, :1448-1475)
       if (C) {
+        uint8_t* act_it = act_mode ? act.data() + (size_t)it * C : nullptr;
         CM_OMP_FOR
         for (long c = 0; c < (long)C; ++c) {
           RV cc[4];
+          const int force = act_mode == 2 ? (int)act_it[c] : -1;
+          bool on;
           if ((size_t)c < vf.size())
-            contact_vf(vf[c], p, cc);
+            on = contact_vf(vf[c], p, cc, force);
           else
-            contact_ee(ee[c - vf.size()], p, cc);
+            on = contact_ee(ee[c - vf.size()], p, cc, force);
+          if (act_mode == 1) act_it[c] = on ? 1 : 0;
           for (int r = 0; r < 4; ++r) rec[4 * c + r] = cc[r];
         }
-        size_t k = 0;
-        while (k < order.size()) {
-          uint32_t vid = rec_id[order[k]];
-          RV a = mk<R>(0, 0, 0);
-          int c = 0;
-          size_t e = k;
-          while (e < order.size() && rec_id[order[e]] == vid) {
-            const RV& val = rec[order[e]];
-            if (!is_zero(val)) {
-              a = a + val;
-              ++c;
-            }
-            ++e;
-          }
-          acc[vid] = acc[vid] + a;
-          cnt[vid] += c;
-          k = e;
-        }
+        accumulate_records(rec_id, order, rec, acc, cnt);
       }
       // FinalCorrectionStep (:1690-1698)
       CM_OMP_FOR

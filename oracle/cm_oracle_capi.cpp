@@ -263,8 +263,36 @@ int64_t orc_get_aabb(void* hp, float* mins, float* maxs, int64_t cap) {
     return n;
   });
 }
-int64_t orc_get_contacts(void* hp, int ee, cm_contact* out, int64_t cap) {
-  return with<float>((Handle*)hp, [&](auto& sim) { return copy_out(ee ? sim.ee : sim.vf, out, cap); });
+// which: 0 VF contacts, 1 EE contacts, 2 VF candidates, 3 EE candidates (the CCD
+// inputs: ownership gate and feature swept-AABB filter passed)
+int64_t orc_get_contacts(void* hp, int which, cm_contact* out, int64_t cap) {
+  return with<float>((Handle*)hp, [&](auto& sim) {
+    const auto& src = which == 3 ? sim.cand_ee : which == 2 ? sim.cand_vf : which ? sim.ee : sim.vf;
+    return copy_out(src, out, cap);
+  });
+}
+
+// Contact activation decisions of the solver: mode 1 records them in the next
+// substeps, 2 replays the ones set with orc_set_act, 0 evaluates them (default).
+int orc_act_mode(void* hp, int mode) {
+  return with<float>((Handle*)hp, [&](auto& sim) {
+    sim.act_mode = mode;
+    sim.act_mismatch = false;
+    return 0;
+  });
+}
+// bytes (iterations * contacts, VF first) of the last solver stage
+int64_t orc_get_act(void* hp, uint8_t* out, int64_t cap) {
+  return with<float>((Handle*)hp, [&](auto& sim) { return copy_out(sim.act, out, cap); });
+}
+int orc_set_act(void* hp, const uint8_t* in, int64_t n) {
+  return with<float>((Handle*)hp, [&](auto& sim) {
+    sim.act.assign(in, in + n);
+    return 0;
+  });
+}
+int orc_act_mismatch(void* hp) {
+  return with<float>((Handle*)hp, [&](auto& sim) { return sim.act_mismatch ? 1 : 0; });
 }
 
 // ---- stage functions on plain arrays ------------------------------------
@@ -345,6 +373,108 @@ int orc_test_ee(const float* x12, float th, cm_contact* c) {
   F3 x[4];
   for (int k = 0; k < 4; ++k) x[k] = mk(x12[3 * k], x12[3 * k + 1], x12[3 * k + 2]);
   return test_ee(x[0], x[1], x[2], x[3], *c, th, 0.000000001f);
+}
+
+// ---- solver building blocks on plain arrays ----------------------------
+// One contact projection (Sim::contact_vf_core / contact_ee_core) in FP32
+// (precision 32) or FP64: p12 = the contact's four positions, w4 = their inverse
+// masses, out12 = the four corrections; force < 0 evaluates the activation.
+// Returns 1 when the constraint was active.
+int orc_contact_project(int ee, int precision, const double* p12, const double* w4,
+                        const cm_contact* k, double th, double* out12, int force) {
+  auto run = [&](auto tag) {
+    typedef decltype(tag) R;
+    typedef V3<R> RV;
+    RV p[4], c[4];
+    R w[4];
+    for (int r = 0; r < 4; ++r) {
+      p[r] = mk<R>(R(p12[3 * r]), R(p12[3 * r + 1]), R(p12[3 * r + 2]));
+      w[r] = R(w4[r]);
+    }
+    bool on = ee ? Sim<R>::contact_ee_core(*k, p, w, R(th), c, force)
+                 : Sim<R>::contact_vf_core(*k, p, w, R(th), c, force);
+    for (int r = 0; r < 4; ++r) {
+      out12[3 * r] = c[r].x;
+      out12[3 * r + 1] = c[r].y;
+      out12[3 * r + 2] = c[r].z;
+    }
+    return on ? 1 : 0;
+  };
+  return precision == 64 ? run(double()) : run(float());
+}
+
+// Per-vertex sum of dynamic correction records (Sim::accumulate_records, FP64):
+// record k belongs to vertex ids[k] with value vals3[3k..3k+2]; acc3 (3N) and cnt
+// (N) are added to.
+void orc_accumulate(int n, int64_t nrec, const uint32_t* ids, const double* vals3, double* acc3,
+                    int32_t* cnt) {
+  typedef V3<double> RV;
+  std::vector<uint32_t> rid(ids, ids + nrec), order;
+  std::vector<RV> rec(nrec), acc(n);
+  std::vector<int> c(n);
+  for (int64_t k = 0; k < nrec; ++k) rec[k] = mk(vals3[3 * k], vals3[3 * k + 1], vals3[3 * k + 2]);
+  for (int i = 0; i < n; ++i) {
+    acc[i] = mk(acc3[3 * i], acc3[3 * i + 1], acc3[3 * i + 2]);
+    c[i] = cnt[i];
+  }
+  Sim<double>::record_order(rid, order);
+  Sim<double>::accumulate_records(rid, order, rec, acc, c);
+  for (int i = 0; i < n; ++i) {
+    acc3[3 * i] = acc[i].x;
+    acc3[3 * i + 1] = acc[i].y;
+    acc3[3 * i + 2] = acc[i].z;
+    cnt[i] = c[i];
+  }
+}
+
+// Long-range attachment correction of one vertex at p3 (Sim::lra_vertex, FP64):
+// pinned vertices fixed[j] at ps3 (positions of all vertices), rest lengths l0[j],
+// flags[j]; a3 / *count receive the summed correction and its count.
+void orc_lra_vertex(const double* p3, int F, const int32_t* fixed, const float* l0,
+                    const uint8_t* flags, int n, const double* ps3, double* a3, int32_t* count) {
+  typedef V3<double> RV;
+  std::vector<RV> ps(n);
+  for (int i = 0; i < n; ++i) ps[i] = mk(ps3[3 * i], ps3[3 * i + 1], ps3[3 * i + 2]);
+  RV a;
+  int c;
+  Sim<double>::lra_vertex(mk(p3[0], p3[1], p3[2]), F, fixed, l0, flags, ps, a, c);
+  a3[0] = a.x;
+  a3[1] = a.y;
+  a3[2] = a.z;
+  *count = c;
+}
+
+// Global and rigid-mode velocity damping (Sim::damp, Physics/CudaPBD.cu:411-458) of
+// vs3 in place at positions ps3 with per-vertex masses, FP64 (precision 64) or FP32.
+void orc_damp(int precision, int n, const double* ps3, double* vs3, const double* mass, float kg,
+              float kd) {
+  auto run = [&](auto tag) {
+    typedef decltype(tag) R;
+    typedef V3<R> RV;
+    Sim<R> sim;
+    sim.n = n;
+    std::memset(&sim.prm, 0, sizeof(sim.prm));
+    sim.prm.global_vel_damp = kg;
+    sim.prm.k_damp = kd;
+    sim.mass.resize(n);
+    std::vector<RV> ps(n), vs(n);
+    for (int i = 0; i < n; ++i) {
+      sim.mass[i] = R(mass[i]);
+      ps[i] = mk<R>(R(ps3[3 * i]), R(ps3[3 * i + 1]), R(ps3[3 * i + 2]));
+      vs[i] = mk<R>(R(vs3[3 * i]), R(vs3[3 * i + 1]), R(vs3[3 * i + 2]));
+    }
+    sim.template damp<R>(ps, vs);
+    for (int i = 0; i < n; ++i) {
+      vs3[3 * i] = vs[i].x;
+      vs3[3 * i + 1] = vs[i].y;
+      vs3[3 * i + 2] = vs[i].z;
+    }
+    return 0;
+  };
+  if (precision == 64)
+    run(double());
+  else
+    run(float());
 }
 
 // ---- differentiable rollout (no reference counterpart) ----

@@ -71,6 +71,15 @@ def lib():
     L.orc_coplanarity_times.argtypes = [vp, vp, vp]
     L.orc_test_vf.argtypes = [vp, C.c_float, vp]
     L.orc_test_ee.argtypes = [vp, C.c_float, vp]
+    L.orc_act_mode.argtypes = [vp, i32]
+    L.orc_get_act.argtypes = [vp, vp, i64]
+    L.orc_get_act.restype = i64
+    L.orc_set_act.argtypes = [vp, vp, i64]
+    L.orc_act_mismatch.argtypes = [vp]
+    L.orc_contact_project.argtypes = [i32, i32, vp, vp, vp, C.c_double, vp, i32]
+    L.orc_accumulate.argtypes = [i32, i64, vp, vp, vp, vp]
+    L.orc_lra_vertex.argtypes = [vp, i32, vp, vp, vp, i32, vp, vp, vp]
+    L.orc_damp.argtypes = [i32, i32, vp, vp, vp, C.c_float, C.c_float]
     _lib = L
     return L
 
@@ -209,11 +218,33 @@ class OracleSim:
         lib().orc_get_aabb(self.h, _p(mn), _p(mx), m)
         return mn, mx
 
-    def contacts(self, ee: bool = False) -> np.ndarray:
-        n = lib().orc_get_contacts(self.h, int(ee), None, 0)
+    def contacts(self, ee: bool = False, candidates: bool = False) -> np.ndarray:
+        """The last substep's VF (or EE) contacts; with candidates, the CCD inputs
+        (candidates past the ownership gate and the feature swept-AABB filter)."""
+        which = int(ee) + (2 if candidates else 0)
+        n = lib().orc_get_contacts(self.h, which, None, 0)
         out = np.zeros(n, dtype=CONTACT_DTYPE)
-        lib().orc_get_contacts(self.h, int(ee), _p(out), n)
+        lib().orc_get_contacts(self.h, which, _p(out), n)
         return out
+
+    def activation_mode(self, mode: int) -> None:
+        """Contact activation decisions of the solver: 1 records them in the following
+        substeps, 2 replays the ones given to set_activations, 0 evaluates them."""
+        lib().orc_act_mode(self.h, int(mode))
+
+    def activations(self) -> np.ndarray:
+        """(iterations, contacts) decisions of the last solver stage, VF contacts first."""
+        n = lib().orc_get_act(self.h, None, 0)
+        out = np.zeros(n, dtype=np.uint8)
+        lib().orc_get_act(self.h, _p(out), n)
+        return out
+
+    def set_activations(self, act) -> None:
+        act = np.ascontiguousarray(act, dtype=np.uint8).ravel()
+        lib().orc_set_act(self.h, _p(act), len(act))
+
+    def activation_mismatch(self) -> bool:
+        return bool(lib().orc_act_mismatch(self.h))
 
     def pairs(self) -> np.ndarray:
         return self.get_i32("pairs").reshape(-1, 2)
@@ -298,3 +329,52 @@ def test_ee(x4: np.ndarray, th: float):
     c = np.zeros(1, dtype=CONTACT_DTYPE)
     hit = lib().orc_test_ee(_p(x4), th, _p(c))
     return bool(hit), c[0]
+
+
+def contact_project(kind: str, p4, w4, contact, th: float, precision: int = 64, force: int = -1):
+    """One VF or EE contact projection (kind "vf" / "ee"): positions p4 (4, 3),
+    inverse masses w4 (4,), a CONTACT_DTYPE record; returns (active, corrections
+    (4, 3))."""
+    p4 = np.ascontiguousarray(p4, dtype=np.float64).reshape(4, 3)
+    w4 = np.ascontiguousarray(w4, dtype=np.float64).reshape(4)
+    c = np.zeros(1, dtype=CONTACT_DTYPE)
+    c[0] = contact
+    out = np.zeros((4, 3))
+    on = lib().orc_contact_project(int(kind == "ee"), precision, _p(p4), _p(w4), _p(c), th,
+                                   _p(out), force)
+    return bool(on), out
+
+
+def accumulate(n: int, ids, vals, acc=None, cnt=None):
+    """Per-vertex sums of correction records (ids (R,), vals (R, 3)) added to acc
+    (n, 3) and cnt (n,), as the solver's dynamic-correction pass."""
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    vals = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1, 3)
+    acc = np.zeros((n, 3)) if acc is None else np.ascontiguousarray(acc, np.float64).copy()
+    cnt = np.zeros(n, np.int32) if cnt is None else np.ascontiguousarray(cnt, np.int32).copy()
+    lib().orc_accumulate(n, len(ids), _p(ids), _p(vals), _p(acc), _p(cnt))
+    return acc, cnt
+
+
+def lra_vertex(p, fixed, l0, flags, positions):
+    """Long-range attachment correction of a vertex at p toward the pinned vertices
+    `fixed` (indices into positions) with rest lengths l0 where flags is set."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    fixed = np.ascontiguousarray(fixed, dtype=np.int32)
+    l0 = np.ascontiguousarray(l0, dtype=np.float32)
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    a = np.zeros(3)
+    c = np.zeros(1, np.int32)
+    lib().orc_lra_vertex(_p(p), len(fixed), _p(fixed), _p(l0), _p(flags), len(pos), _p(pos),
+                         _p(a), _p(c))
+    return a, int(c[0])
+
+
+def damp(positions, velocities, masses, kg: float, kd: float, precision: int = 64):
+    """Velocities after global (kg) and rigid-mode (kd) damping."""
+    ps = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+    vs = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1, 3).copy()
+    m = np.ascontiguousarray(masses, dtype=np.float64)
+    lib().orc_damp(precision, len(ps), _p(ps), _p(vs), _p(m), kg, kd)
+    return vs

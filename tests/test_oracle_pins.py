@@ -1,0 +1,386 @@
+"""Pins the oracle's solver and narrow-phase building blocks with closed forms,
+invariants and brute force (FP64 unless stated):
+
+* VF contact projection (Physics/CudaPBD.cu:1212-1311): a vertex closer than the
+  thickness to a triangle point q = sum b_i p_i is moved, with q, so that
+  |p - q| = thickness exactly, the corrections are weighted by
+  w / (w + sum w_i b_i^2), and linear momentum is conserved;
+* EE contact projection (:1313-1415): the two edge points end exactly one
+  thickness apart, weights w_k b_k / sum w_j b_j^2, momentum conserved;
+* dynamic-correction accumulation (:711-720, :1448-1475): per vertex the sum and
+  count of the non-zero records only, in record order (scatter-add brute force);
+* long-range attachments (:1571-1592, :1656-1688): one attachment pulls the vertex
+  exactly onto the sphere of radius l0 around its pin; inside the whole solver the
+  correction is averaged with the vertex's static slots;
+* rigid-mode damping (:411-458, :1934-1990): a rigid velocity field is left
+  unchanged, a pure expansion is scaled by 1 - k_damp, global damping by 1 - kg;
+* world clamp (Physics/CudaDeformableUtils.cu:251-269): after a substep the
+  predicted positions are the carried positions clamped to the box;
+* candidate expansion and the feature swept-AABB filter
+  (Collision/CudaCollisionDetection.cu:101-187, Collision/CudaPrimitiveTests.cu:427-490):
+  the CCD inputs equal a brute-force enumeration from the pair list, ownership
+  computed from the mesh and the particle boxes;
+* the reference's multiply-and-mask bit spread (Core/CudaUtils.cu:278-287) equals
+  the oracle's Morton interleave for all 2^21 inputs.
+
+Each check is also shown to reject a planted mistake (a flipped sign, swapped
+weights, a dropped term, zero records counted), applied to the oracle's output."""
+import numpy as np
+import pytest
+
+from licenta_amd import configs
+from licenta_amd._abi import CONTACT_DTYPE
+
+
+def _contact(v, w):
+    c = np.zeros(1, CONTACT_DTYPE)[0]
+    c["v"] = v
+    c["w"] = w
+    c["t"] = 0.5
+    c["n"] = (0.0, 1.0, 0.0)
+    return c
+
+
+def _vf_setup(bary, depth, rng):
+    tri = rng.uniform(-1, 1, (3, 3))
+    q = bary @ tri
+    nrm = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+    nrm /= np.linalg.norm(nrm)
+    p = q + depth * nrm
+    return np.vstack([p, tri]), nrm
+
+
+@pytest.mark.parametrize("bary", [(1 / 3, 1 / 3, 1 / 3), (0.2, 0.3, 0.5), (0.0, 0.25, 0.75)])
+def test_vf_projection_restores_thickness(oracle_mod, bary):
+    rng = np.random.default_rng(1)
+    b = np.array(bary, np.float32).astype(np.float64)  # contact weights are FP32
+    th, d = 0.01, 0.004
+    p4, nrm = _vf_setup(b, d, rng)
+    w4 = np.array([2.0, 0.5, 1.0, 3.0])
+    c = _contact([0, 1, 2, 3], [1.0, -b[0], -b[1], -b[2]])
+    on, cor = oracle_mod.contact_project("vf", p4, w4, c, th)
+    assert on
+    s = w4[0] + np.sum(w4[1:] * b * b)
+    want = np.vstack([(th - d) * w4[0] / s * nrm,
+                      -((th - d) * w4[1:, None] * b[:, None] / s) * nrm])
+    np.testing.assert_allclose(cor, want, rtol=1e-12, atol=1e-15)
+
+    def separation(c4):
+        q4 = p4 + c4
+        return np.linalg.norm(q4[0] - b @ q4[1:])
+
+    assert abs(separation(cor) - th) < 1e-12
+    m = 1.0 / w4
+    # momentum: -s n + s n sum(b), zero up to the FP32 rounding of the weights
+    np.testing.assert_allclose((m[:, None] * cor).sum(0), 0.0,
+                               atol=1e-12 + abs(1 - b.sum()) * np.abs(cor[0]).max() / w4[0])
+    # planted mistakes: a flipped sign, weights of two vertices swapped, a dropped
+    # triangle term each break the closed form
+    assert abs(separation(-cor) - th) > 1e-4
+    swapped = cor.copy()
+    swapped[[1, 3]] = cor[[1, 3]] * (w4[[3, 1]] / w4[[1, 3]])[:, None]
+    assert abs(separation(swapped) - th) > 1e-5 or \
+        np.abs((m[:, None] * swapped).sum(0)).max() > 1e-6
+    dropped = cor.copy()
+    dropped[2] = 0
+    assert abs(separation(dropped) - th) > 1e-5
+
+
+def test_vf_projection_inactive_and_clamped(oracle_mod):
+    rng = np.random.default_rng(2)
+    th = 0.01
+    # beyond the thickness: no correction, inactive
+    p4, _ = _vf_setup(np.array([0.3, 0.3, 0.4]), 0.02, rng)
+    c = _contact([0, 1, 2, 3], [1.0, -0.3, -0.3, -0.4])
+    on, cor = oracle_mod.contact_project("vf", p4, np.ones(4), c, th)
+    assert not on and not cor.any()
+    # forced on (activation replay): the projection of the same contact
+    on, cor = oracle_mod.contact_project("vf", p4, np.ones(4), c, th, force=1)
+    assert on and cor.any()
+    # weights outside [0, 1] are clamped: a positive weight gives barycentric 0
+    bc = np.array([0.0, 0.375, 1.0])
+    p4, nrm = _vf_setup(bc, 0.003, rng)
+    c = _contact([0, 1, 2, 3], [1.0, 0.125, -0.375, -1.5])  # clamps to bc
+    on, cor = oracle_mod.contact_project("vf", p4, np.ones(4), c, th)
+    q = bc @ p4[1:]
+    n = (p4[0] - q) / np.linalg.norm(p4[0] - q)
+    s = 1.0 + np.sum(bc * bc)
+    cc = np.linalg.norm(p4[0] - q) - th
+    assert on
+    np.testing.assert_allclose(cor[0], -cc / s * n, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(cor[1], 0.0, atol=0)
+    np.testing.assert_allclose(cor[3], cc / s * bc[2] * n, rtol=1e-12, atol=1e-15)
+    # all inverse masses zero: no correction
+    on, cor = oracle_mod.contact_project("vf", p4, np.zeros(4), c, th)
+    assert not cor.any()
+
+
+@pytest.mark.parametrize("a,b", [(0.5, 0.5), (0.3, 0.8), (0.0, 1.0)])
+def test_ee_projection_restores_thickness(oracle_mod, a, b):
+    rng = np.random.default_rng(3)
+    th, d = 0.02, 0.007
+    # edge 1 along x through the origin, edge 2 along z, d above along y
+    a, b = float(np.float32(a)), float(np.float32(b))
+    e1 = np.array([[-a, 0, 0], [1 - a, 0, 0]])
+    e2 = np.array([[0, -d, -b], [0, -d, 1 - b]])
+    rot = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    p4 = np.vstack([e1, e2]) @ rot.T + rng.normal(size=3)
+    w4 = np.array([1.5, 0.25, 2.0, 0.75])
+    c = _contact([0, 1, 2, 3], [1 - a, a, -(1 - b), -b])
+    on, cor = oracle_mod.contact_project("ee", p4, w4, c, th)
+    assert on
+    bw = np.array([1 - a, a, 1 - b, b])
+
+    def gap(c4):
+        q4 = p4 + c4
+        return np.linalg.norm(bw[0] * q4[0] + bw[1] * q4[1] - bw[2] * q4[2] - bw[3] * q4[3])
+
+    assert abs(gap(cor) - th) < 1e-12
+    nrm = (bw[0] * p4[0] + bw[1] * p4[1] - bw[2] * p4[2] - bw[3] * p4[3]) / d
+    s = np.sum(w4 * bw * bw)
+    sign = np.array([1, 1, -1, -1])
+    want = (th - d) / s * (w4 * bw * sign)[:, None] * nrm
+    np.testing.assert_allclose(cor, want, rtol=1e-12, atol=1e-15)
+    m = 1.0 / w4
+    np.testing.assert_allclose((m[:, None] * cor).sum(0), 0.0, atol=1e-12)
+    assert abs(gap(-cor) - th) > 1e-4
+    assert abs(gap(cor * np.array([1, 1, 0, 0])[:, None]) - th) > 1e-4
+
+
+def test_dynamic_accumulation_counts_nonzero_only(oracle_mod):
+    rng = np.random.default_rng(4)
+    n, r = 40, 600
+    ids = rng.integers(0, n, r).astype(np.uint32)
+    vals = rng.normal(size=(r, 3))
+    zero = rng.random(r) < 0.35
+    vals[zero] = 0.0
+    vals[np.nonzero(zero)[0][::3]] = -0.0  # a negative zero is a zero record too
+    ids[ids == 7] = 8  # vertex 7: no records; vertex 9: only zero records
+    vals[ids == 9] = 0.0
+    acc0 = rng.normal(size=(n, 3))
+    cnt0 = rng.integers(0, 5, n).astype(np.int32)
+    acc, cnt = oracle_mod.accumulate(n, ids, vals, acc0, cnt0)
+    # brute force: unbuffered scatter-add in record order, count of non-zero rows
+    nz = np.any(vals != 0, axis=1)
+    want = np.zeros((n, 3))
+    np.add.at(want, ids[nz], vals[nz])
+    assert np.array_equal(acc, acc0 + want)
+    want_cnt = cnt0 + np.bincount(ids[nz], minlength=n)
+    assert np.array_equal(cnt, want_cnt)
+    assert cnt[7] == cnt0[7] and cnt[9] == cnt0[9] and np.array_equal(acc[9], acc0[9])
+    # planted mistake: counting every record differs wherever a zero record landed
+    assert not np.array_equal(cnt0 + np.bincount(ids, minlength=n), cnt)
+
+
+def test_lra_vertex_closed_form(oracle_mod):
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-1, 1, (6, 3))
+    p = np.array([0.3, 2.0, -0.4])
+    l0 = np.float32(1.25)
+    a, c = oracle_mod.lra_vertex(p, [2], [l0], [1], pos)
+    assert c == 1
+    np.testing.assert_allclose(np.linalg.norm(p + a - pos[2]), np.float64(l0), rtol=1e-12)
+    d = p - pos[2]
+    assert np.dot(a, d) < 0 and np.linalg.norm(np.cross(a, d)) < 1e-12
+    # two pins, one flagged off; and none flagged
+    a2, c2 = oracle_mod.lra_vertex(p, [2, 4, 5], [l0, 0.5, 0.7], [1, 0, 1], pos)
+    a5, _ = oracle_mod.lra_vertex(p, [5], [0.7], [1], pos)
+    assert c2 == 2
+    np.testing.assert_allclose(a2, a + a5, rtol=1e-12, atol=1e-15)
+    a0, c0 = oracle_mod.lra_vertex(p, [2, 4], [l0, 0.5], [0, 0], pos)
+    assert c0 == 0 and not a0.any()
+
+
+def test_lra_in_solver_averaged_with_static_slots(oracle_mod):
+    """One triangle, vertex 0 pinned, stretch stiffness 0 (its slots count but move
+    nothing), LRA on: the free vertex 1, pulled out past its attachment length, moves
+    by the attachment correction over (2 slots + 1); vertex 2 stays inside and only
+    its slots count."""
+    p = configs.default_params()
+    p.update(timestep=1.0 / 60.0, gravity=(0.0, 0.0, 0.0), k_stretch=0.0, k_bend=0.0,
+             thickness=0.001, solver_iterations=1, strain_limit=0.1,
+             flags=configs.F_LRA, ref_compat=configs.RC_ALL)
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 1]], np.float32)
+    sc = dict(params=p, xyz=xyz, tri=np.array([[0, 1, 2]], np.uint32),
+              inv_mass=np.array([0, 1, 1], np.float32), pinned=None, dist_pairs=None,
+              k_dist=None, k_bend=None)
+    o = oracle_mod.OracleSim(sc, 64)
+    l0 = o.get_f32("lra_l0").reshape(3, 1)
+    assert l0[1, 0] == np.float32(1.0) * np.float32(1.1)
+    x = xyz.astype(np.float64).copy()
+    x[1] = [2.0, 0.0, 0.0]
+    o.set_state(x, np.zeros((3, 3)))
+    o.step(1)
+    xs, _, _ = o.state()
+    corr = -(2.0 - np.float64(l0[1, 0])) * np.array([1.0, 0, 0])
+    np.testing.assert_allclose(xs[1], x[1] + corr / 3.0, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(xs[2], x[2], atol=1e-12)
+    np.testing.assert_allclose(xs[0], x[0], atol=0)
+    # planted mistake: without the static slots in the count the step is 3x as long
+    assert np.abs(xs[1] - (x[1] + corr)).max() > 0.5
+
+
+def _cloud(rng, n=50):
+    pos = rng.uniform(-1, 1, (n, 3))
+    mass = rng.uniform(0.5, 3.0, n)
+    return pos, mass
+
+
+def test_rigid_mode_damping_keeps_rigid_motion(oracle_mod):
+    rng = np.random.default_rng(6)
+    pos, mass = _cloud(rng)
+    xcm = (mass[:, None] * pos).sum(0) / mass.sum()
+    V, om = rng.normal(size=3), rng.normal(size=3)
+    v = V + np.cross(om, pos - xcm)
+    out = oracle_mod.damp(pos, v, mass, 0.0, 0.375)
+    np.testing.assert_allclose(out, v, rtol=0, atol=1e-12)
+    # a pure expansion about the centre of mass has no momentum and no angular
+    # momentum: scaled by 1 - k_damp
+    e = 0.7 * (pos - xcm)
+    out = oracle_mod.damp(pos, e, mass, 0.0, 0.375)
+    np.testing.assert_allclose(out, 0.625 * e, rtol=0, atol=1e-12)
+    # linear in the velocity field: rigid part kept, deformation damped
+    out = oracle_mod.damp(pos, v + e, mass, 0.0, 0.375)
+    np.testing.assert_allclose(out, v + 0.625 * e, rtol=0, atol=1e-12)
+    # global damping scales everything first
+    out = oracle_mod.damp(pos, v + e, mass, 0.25, 0.0)
+    np.testing.assert_allclose(out, 0.75 * (v + e), rtol=0, atol=1e-12)
+    # FP32 form: the same within single-precision rounding
+    out32 = oracle_mod.damp(pos, v + e, mass, 0.0, 0.375, precision=32)
+    np.testing.assert_allclose(out32, v + 0.625 * e, rtol=0, atol=2e-5)
+    # planted mistake: with the inertia tensor's off-diagonal sign flipped the rigid
+    # field would not be reproduced (checked on the same data in numpy)
+    r = pos - xcm
+    L = (mass[:, None] * np.cross(r, v)).sum(0)
+    I_ok = sum(m * (np.dot(q, q) * np.eye(3) - np.outer(q, q)) for m, q in zip(mass, r))
+    I_bad = I_ok.copy()
+    I_bad[~np.eye(3, dtype=bool)] *= -1
+    assert np.abs(np.linalg.solve(I_ok, L) - om).max() < 1e-10
+    assert np.abs(np.linalg.solve(I_bad, L) - om).max() > 1e-3
+
+
+def test_world_clamp_clamps_predicted_only(oracle_mod):
+    n = 10
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(-1, 3).astype(np.float64).copy()
+    xyz[:, 0] += 0.96  # the cloth crosses x = 1
+    xyz[:, 1] = 0.002
+    lo, hi = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    sc = dict(sc, xyz=xyz.astype(np.float32), pinned=None,
+              inv_mass=np.full(n * n, 1000.0, np.float32),
+              params=dict(sc["params"], flags=configs.F_WORLD_CLAMP, world_min=lo, world_max=hi))
+    for prec in (32, 64):
+        o = oracle_mod.OracleSim(sc, prec)
+        o.step(3)
+        x, p, _ = o.state()
+        assert (x[:, 0] > 1.0).any() and (x[:, 1] < 0.0).any()  # carried state unclamped
+        assert np.array_equal(p, np.clip(x, lo, hi))
+
+
+def _folded(n=16):
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    h = n // 2
+    for i in range(h, n):
+        src = 2 * h - 1 - i
+        xyz[i, :, 0] = xyz[src, :, 0] + 0.002
+        xyz[i, :, 1] = 0.003 + 0.0005 * (i - h)
+    sc["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    sc["params"] = dict(sc["params"], thickness=0.004)
+    sc["pinned"] = None
+    sc["inv_mass"] = np.full(n * n, 1000.0, np.float32)
+    return sc
+
+
+def _owners(tri):
+    """Lowest-index owner triangle of every vertex and every edge."""
+    vown, eown = {}, {}
+    for i, (a, b, c) in enumerate(tri):
+        for v in (a, b, c):
+            vown.setdefault(int(v), i)
+        for u, w in ((a, b), (b, c), (c, a)):
+            eown.setdefault((int(min(u, w)), int(max(u, w))), i)
+    return vown, eown
+
+
+def test_candidates_and_feature_filter_brute_force(oracle_mod):
+    sc = _folded()
+    prm = sc["params"]
+    o = oracle_mod.OracleSim(sc, 32)
+    rng = np.random.default_rng(7)
+    v0 = rng.normal(0, 0.05, (len(sc["xyz"]), 3)).astype(np.float32)
+    v0.reshape(16, 16, 3)[8:, :, 1] = -0.3
+    x0 = sc["xyz"].astype(np.float32)
+    o.set_state(x0.astype(np.float64), v0.astype(np.float64))
+    o.step(1)
+    # the positions of the collision stage: prediction without damping, in FP32
+    dt = np.float32(prm["timestep"])
+    g = np.asarray(prm["gravity"], np.float32)
+    vel = v0 + dt * g
+    pred = x0 + vel * dt
+    th = np.float32(prm["thickness"])
+    pmin = np.minimum(pred, x0) - th
+    pmax = np.maximum(pred, x0) + th
+    init = sc["tri"].astype(np.int64)
+    vown, eown = _owners(init)
+    tri_id = {tuple(t): i for i, t in enumerate(init.tolist())}
+    tri = o.get_i32("tri").reshape(-1, 4)  # sorted order, feature bits in column 3
+    pairs = o.pairs()
+    assert len(pairs) > 50
+
+    def overlap(amin, amax, bmin, bmax):
+        return bool(np.all(amin <= bmax) and np.all(bmin <= amax))
+
+    def box(vs):
+        return pmin[list(vs)].min(0), pmax[list(vs)].max(0)
+
+    want_vf, want_ee = [], []
+    for i, j in pairs:
+        A, B = tri[i, :3].tolist(), tri[j, :3].tolist()
+        ia, ib = tri_id[tuple(A)], tri_id[tuple(B)]
+        for P, Q, owner in ((A, B, ia), (B, A, ib)):
+            for v in P:
+                if vown[v] == owner and overlap(pmin[v], pmax[v], *box(Q)):
+                    want_vf.append((v, *Q))
+    for i, j in pairs:
+        A, B = tri[i, :3].tolist(), tri[j, :3].tolist()
+        ia, ib = tri_id[tuple(A)], tri_id[tuple(B)]
+        edges = lambda T: ((T[0], T[1]), (T[0], T[2]), (T[1], T[2]))
+        for eb in edges(B):
+            for ea in edges(A):
+                if eown[(min(ea), max(ea))] != ia or eown[(min(eb), max(eb))] != ib:
+                    continue
+                if overlap(*box(ea), *box(eb)):
+                    want_ee.append((*ea, *eb))
+    got_vf = [tuple(c) for c in o.contacts(False, candidates=True)["v"].tolist()]
+    got_ee = [tuple(c) for c in o.contacts(True, candidates=True)["v"].tolist()]
+    assert got_vf == want_vf and got_ee == want_ee
+    assert len(got_vf) > 20 and len(got_ee) > 20
+    # the contacts are a subsequence of the candidates (CCD only removes)
+    vf = [tuple(c) for c in o.contacts(False)["v"].tolist()]
+    it = iter(got_vf)
+    assert all(c in it for c in vf)
+    # planted mistake: an exclusive overlap test, or ownership ignored, changes the set
+    ignoring = sum(1 for i, j in pairs for _ in range(6))
+    assert ignoring != len(want_vf)
+
+
+def test_expandby2_multiply_mask_equals_interleave(oracle_mod):
+    """The reference's ExpandBy2 steps (multiply by 2^32+1, 2^16+1, 2^8+1, 2^4+1,
+    2^2+1, each followed by a mask) against the oracle's 63-bit Morton code for every
+    21-bit coordinate value on each axis."""
+    q = np.arange(1 << 21, dtype=np.uint64)
+    v = q.copy()
+    for mul, mask in ((0x000100000001, 0xFFFF00000000FFFF), (0x000000010001, 0x00FF0000FF0000FF),
+                      (0x000000000101, 0xF00F00F00F00F00F), (0x000000000011, 0x30C30C30C30C30C3),
+                      (0x000000000005, 0x9249249249249249)):
+        v = (v * np.uint64(mul)) & np.uint64(mask)
+    coord = (q.astype(np.float64) / (1 << 21)).astype(np.float32)  # exact: quantises to q
+    zeros = np.zeros_like(coord)
+    for axis, shift in ((0, 2), (1, 1), (2, 0)):
+        c3 = np.stack([coord if a == axis else zeros for a in range(3)], 1)
+        k = oracle_mod.morton63(c3)
+        assert np.array_equal(k, v << np.uint64(shift)), axis
+    # planted mistake: a dropped step leaves bits unspread
+    w = (q * np.uint64(0x000100000001)) & np.uint64(0xFFFF00000000FFFF)
+    assert not np.array_equal(w, v)
