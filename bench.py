@@ -4,14 +4,22 @@ Workload: one 256x256-vertex cloth (65,536 vertices, 130,050 triangles, 260,610
 structural+shear distance constraints, 194,565 dihedral bends) dropped from 0.5 m
 onto a static sphere of radius 0.3 m, thickness 0.002 m, dt 1/120 s with 2
 substeps per frame (substep dt 1/240 s), 15 Jacobi iterations, 30-bit Morton codes
-in the centroid box, LBVH self-collision with VF and EE contacts every substep.
-A step is one substep; the default run times the full 600 frames (1,200 substeps).
+in the centroid box, LBVH self-collision with VF and EE contacts every substep,
+600 frames (1,200 substeps).
 
-Prints one JSON line. ``roofline`` covers the dominant kernel (the fused Jacobi
-iteration, 15 launches per substep) with its algorithmic bytes per launch
-(36 N + 16 S + 24 Bd + 48 C) over its average launch time measured with HIP events
-on the body's stream; ``cpu_baseline`` times the FP32 CPU oracle on a bounded
-sample of the same workload on this host's cores.
+A step is one substep. The drop runs from its initial state; the first
+1,200 - steps substeps are an untimed pre-roll and the last `--steps` substeps,
+where the cloth lies draped on the sphere with its densest self-contact, are the
+timed window (`value`). `full_run` gives the whole 1,200-substep drop's rate and
+`early_window` its first 20 substeps. With --steps 1200 or more the whole run is
+timed.
+
+Prints one JSON line. ``roofline`` covers the window's dominant kernel (the BVH
+traversal or the fused Jacobi iteration, whichever takes more time per substep)
+with its algorithmic bytes per launch over its average launch time measured live
+with HIP events on the body's stream; ``roofline_other`` gives the other one.
+``cpu_baseline`` times the FP32 CPU oracle seeded with the GPU's state entering the
+window, on a bounded sample of the same substeps on this host's cores.
 """
 from __future__ import annotations
 
@@ -30,47 +38,96 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "PBD substeps/s at 256^2 self-colliding cloth"
 
 
+FRAMES_SUBSTEPS = 1200  # 600 frames at 2 substeps per frame
+EARLY_WINDOW = 20
+KERNEL_SOLVE, KERNEL_TRAVERSE = 0, 2
+WARN_CANDIDATE_CAP = 1 << 4
+
+
+def _timed(body, substeps: int) -> float:
+    """Wall seconds of `substeps` substeps (graph replays) with the state in HBM."""
+    if substeps <= 0:
+        return 0.0
+    body.sync()
+    t0 = time.perf_counter()
+    body.fixed_update(substeps)
+    body.sync()
+    return time.perf_counter() - t0
+
+
+def _traffic_profiles(size: int) -> dict:
+    """HBM bytes per launch measured by rocprofv3 counters (profiles/*.json), scaled
+    by a plain copy's reading; (None, None) where no profile exists."""
+    out = {}
+    for key, name in (("traverse", "k_traverse_sphere_drop_256_traffic.json"),
+                      ("solve", "k_solve_sphere_drop_256_traffic.json")):
+        tf = os.path.join(ROOT, "profiles", name)
+        if size == 256 and os.path.exists(tf):
+            tj = json.load(open(tf))
+            out[key] = (tj.get("bytes_per_launch"),
+                        f"rocprofv3 FETCH_SIZE x {tj.get('fetch_scale')} + WRITE_SIZE x "
+                        f"{tj.get('write_scale')} (256 MiB copy calibration), {tj.get('state')}")
+        else:
+            out[key] = (None, None)
+    return out
+
+
 def algorithmic_bytes(n, s, bd, t, iters, contacts, pairs):
     per_iter = 36 * n + 16 * s + 24 * bd + 48 * contacts
     substep = 112 * n + iters * per_iter + 300 * t + 32 * n + 8 * pairs
     return per_iter, substep
 
 
-def cpu_baseline(scene, budget_s: float):
-    """The FP32 oracle (the C++ restatement of the reference path) on this host:
-    all cores (OpenMP over vertices, constraints, triangles, leaves, pairs and
-    candidates; results identical to one thread), then one thread, each on the
-    first substeps of the same workload within the time budget."""
+def cpu_baseline(scene, budget_s: float, state: dict | None = None):
+    """The FP32 oracle (the C++ restatement of the reference path) on this host, on
+    the substeps the GPU timed: seeded with the GPU's state entering the window, it
+    steps substep after substep on all cores (OpenMP over vertices, constraints,
+    triangles, leaves, pairs and candidates; results identical to one thread) until
+    the time budget is spent, then on one thread when one substep fits the budget."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the CPU restatement, used only as the reported baseline
 
-    def run(threads, budget):
-        used = oracle.OracleSim.set_threads(threads)
+    def fresh():
         o = oracle.OracleSim(scene, 32)
+        if state is not None:
+            o.set_state(state["x"], state["v"], state["p"], state["tri"])
+            o.set_parents(state["parents"])
+        return o
+
+    def run(threads, budget, max_steps=64):
+        used = oracle.OracleSim.set_threads(threads)
+        o = fresh()
         done, elapsed = 0, 0.0
-        while elapsed < budget and done < 64:
+        while elapsed < budget and done < max_steps:
             elapsed += o.time_steps(1)
             done += 1
         return used, done, elapsed
 
+    where = (f"substeps {state['substep'] + 1}.. of the drop, seeded with the GPU's state "
+             f"entering the timed window" if state is not None else "first substeps of the drop")
     default = oracle.OracleSim.set_threads(0)
     try:
         cores, done, elapsed = run(default, budget_s)
-        _, done1, elapsed1 = run(1, budget_s / 2)
+        single = None
+        if elapsed / done * cores <= 2 * budget_s:
+            _, done1, elapsed1 = run(1, budget_s / 2)
+            single = dict(value=done1 / elapsed1, unit="substeps/s", cores=1,
+                          sample=f"{done1} substeps, 1 thread, {elapsed1:.1f} s")
     finally:
         oracle.OracleSim.set_threads(default)
     return dict(value=done / elapsed, unit="substeps/s", cores=cores, kind="port",
-                sample=f"first {done} substeps of the same 256x256 sphere-drop workload, FP32 "
+                sample=f"{done} substeps of the 256x256 sphere-drop workload ({where}), FP32 "
                        f"oracle (C++ restatement of the reference path), {cores} OpenMP "
                        f"threads, {elapsed:.1f} s",
-                single_thread=dict(value=done1 / elapsed1, unit="substeps/s", cores=1,
-                                   sample=f"first {done1} substeps, 1 thread, {elapsed1:.1f} s"))
+                single_thread=single)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1200)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="substeps timed: the last ones of the 1,200-substep drop (all of "
+                         "them when 1,200 or more), after an untimed pre-roll from its start")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -82,7 +139,9 @@ def main():
                          "the late state prefers other shapes); the batched leg always tunes")
     ap.add_argument("--batch-cloths", type=int, default=512,
                     help="cloths of the batched-vs-plain leg (0 skips it)")
-    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--batch-steps", type=int, default=300,
+                    help="substeps of the batched-vs-plain leg (300 = the workload's 300 "
+                         "frames at one substep per frame)")
     ap.add_argument("--large-size", type=int, default=1024,
                     help="side of the crumpled large cloth (0 skips that leg)")
     ap.add_argument("--large-steps", type=int, default=200)
@@ -112,45 +171,77 @@ def main():
 
     scene = configs.config_scene(1, args.size)
     body = ClothBody.from_scene(scene)
-    body.fixed_update(args.warmup)
+    x0 = scene["xyz"].astype(np.float32)
+    tri0 = body.buffer("tri").copy()
+    body.fixed_update(args.warmup)  # untimed: graph capture, caches
     body.sync()
     tuned = None
     if args.autotune:
         tuned = body.autotune(3)  # launch shapes only
-        body.fixed_update(1)  # captures the tuned substep graph
-        body.sync()
-    t0 = time.perf_counter()
-    body.fixed_update(args.steps)
+    # restart the drop from its initial state; the run is the workload's 600 frames
+    body.set_state(x0, np.zeros_like(x0), tri=tri0)
     body.sync()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    run_len = max(FRAMES_SUBSTEPS, args.steps)
+    preroll = run_len - args.steps
+    early_n = min(EARLY_WINDOW, preroll)
+    t_early = _timed(body, early_n)
+    t_rest = _timed(body, preroll - early_n)
+    counts_pre = body.counts()
+    warn_pre = body.warnings()
+    # the state entering the window, for the CPU baseline on the same substeps
+    window_state = None if args.no_cpu else dict(
+        x=body.buffer("x")[:, :3].astype(np.float64), p=body.buffer("p")[:, :3].astype(np.float64),
+        v=body.buffer("v")[:, :3].astype(np.float64), tri=body.buffer("tri").copy(),
+        parents=body.buffer("parents").copy(), substep=preroll)
+    # the timed window: the last `steps` substeps of the run (the draped, contact-heavy
+    # state when steps is below the run's length)
+    t_win = _timed(body, args.steps)
+    ms_per_step = t_win * 1e3 / args.steps
     counts = body.counts()
     warn = body.warnings()
-    # per-stage split of the last substeps (eager launches between HIP events)
+    # per-stage split of the window's state (eager launches between HIP events, 10 more
+    # substeps)
     stage_names = ("predict", "triangles", "sort", "tree", "pairs", "narrow", "solver", "integrate")
     stage = (C.c_float * 8)()
     nstage = 10
     if lib().cm_stage_times(body.h, nstage, stage):
         raise RuntimeError("cm_stage_times failed")
     stage_ms = {k: round(stage[i] / nstage, 4) for i, k in enumerate(stage_names)}
-    # dominant kernel, timed live with HIP events on the body's stream
-    avg = C.c_float()
-    rc = lib().cm_time_kernel(body.h, 0, args.kernel_reps, C.byref(avg))
-    if rc:
-        raise RuntimeError("cm_time_kernel failed")
+    counts_after = body.counts()
     iters = scene["params"]["solver_iterations"]
-    contacts = counts["vf"] + counts["ee"]
-    per_iter, per_substep = algorithmic_bytes(counts["n"], counts["s"], counts["b"], counts["t"],
-                                              iters, contacts, counts["pairs"])
+    contacts = counts_after["vf"] + counts_after["ee"]
+    # live HIP-event timing of the window's two largest kernels on its final state:
+    # the BVH traversal (one launch per substep) and the Jacobi iteration (15 per substep)
+    avg = C.c_float()
+    if lib().cm_time_kernel(body.h, KERNEL_TRAVERSE, args.kernel_reps, C.byref(avg)):
+        raise RuntimeError("cm_time_kernel(traverse) failed")
+    trav_ms = float(avg.value)
+    if lib().cm_time_kernel(body.h, KERNEL_SOLVE, args.kernel_reps, C.byref(avg)):
+        raise RuntimeError("cm_time_kernel(solve) failed")
     solve_ms = float(avg.value)
-    achieved = per_iter / (solve_ms * 1e-3) / 1e9
-    traffic, traffic_note = None, None
-    tf = os.path.join(ROOT, "profiles", "k_solve_sphere_drop_256_traffic.json")
-    if os.path.exists(tf) and args.size == 256:
-        tj = json.load(open(tf))
-        traffic = tj.get("bytes_per_launch")
-        traffic_note = (f"rocprofv3 FETCH_SIZE x {tj.get('fetch_scale')} + WRITE_SIZE x "
-                        f"{tj.get('write_scale')} (256 MiB copy calibration), {tj.get('state')}")
+    n, s, bd, t = counts_after["n"], counts_after["s"], counts_after["b"], counts_after["t"]
+    per_iter, per_substep = algorithmic_bytes(n, s, bd, t, iters, contacts, counts_after["pairs"])
+    trav_bytes = 88 * t + 8 * counts_after["pairs"]
+    trav_gbs = trav_bytes / (trav_ms * 1e-3) / 1e9
+    solve_gbs = per_iter / (solve_ms * 1e-3) / 1e9
+    # the dominant kernel of the window by time per substep
+    trav_share, solve_share = trav_ms, solve_ms * iters
+    dominant = "traverse" if trav_share >= solve_share else "solve"
+    prof = _traffic_profiles(args.size)
+    roof_trav = dict(bound="hbm", achieved=trav_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=trav_gbs / HBM_PEAK_GBS, traffic=prof["traverse"][0],
+                     traffic_source=prof["traverse"][1],
+                     kernel="k_traverse (BVH traversal, one launch per substep)",
+                     bytes_per_launch=trav_bytes, avg_launch_ms=trav_ms,
+                     ms_per_substep=trav_share)
+    roof_solve = dict(bound="hbm", achieved=solve_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=solve_gbs / HBM_PEAK_GBS, traffic=prof["solve"][0],
+                      traffic_source=prof["solve"][1],
+                      kernel="k_solve (fused Jacobi iteration, 15 launches per substep)",
+                      bytes_per_launch=per_iter, avg_launch_ms=solve_ms,
+                      ms_per_substep=solve_share)
+    roof = roof_trav if dominant == "traverse" else roof_solve
+    full_s = t_early + t_rest + t_win
     out = dict(
         metric=METRIC,
         value=1e3 / ms_per_step,
@@ -164,16 +255,23 @@ def main():
         vs_baseline=None,
         dtype="f32",
         data="synthetic",
-        config=dict(workload=f"sphere_drop_{args.size}x{args.size}", vertices=counts["n"],
-                    triangles=counts["t"], distance_constraints=counts["s"],
-                    bend_constraints=counts["b"], iterations=iters, substep_dt=1 / 240,
-                    morton_bits=30, parallelism="single cloth, one GPU"),
-        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_note,
-                      kernel="k_solve (fused Jacobi iteration)", bytes_per_launch=per_iter,
-                      avg_launch_ms=solve_ms),
-        vertex_updates_per_s=counts["n"] * 1e3 / ms_per_step,
+        config=dict(workload=f"sphere_drop_{args.size}x{args.size}", vertices=n, triangles=t,
+                    distance_constraints=s, bend_constraints=bd, iterations=iters,
+                    substep_dt=1 / 240, morton_bits=30, frames=run_len // 2,
+                    timed_window=f"substeps {preroll + 1}-{run_len} of {run_len}",
+                    parallelism="single cloth, one GPU"),
+        roofline=roof,
+        roofline_other=roof_solve if dominant == "traverse" else roof_trav,
+        vertex_updates_per_s=n * 1e3 / ms_per_step,
         substep_roofline_frac=per_substep / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        full_run=dict(substeps=run_len, seconds=full_s, substeps_per_s=run_len / full_s,
+                      ms_per_substep=full_s * 1e3 / run_len,
+                      note="whole 600-frame drop from the initial state, wall clock with a "
+                           "sync between the early window, the rest and the timed window"),
+        early_window=dict(substeps=early_n, substeps_per_s=(early_n / t_early) if early_n else None,
+                          note="first substeps of the drop (flat cloth above the sphere)"),
+        window_start=dict(pairs=counts_pre["pairs"], vf=counts_pre["vf"], ee=counts_pre["ee"],
+                          warnings=warn_pre),
         last_substep=dict(pairs=counts["pairs"], vf=counts["vf"], ee=counts["ee"],
                           warnings=warn),
         autotuned=tuned,
@@ -193,7 +291,8 @@ def main():
         if args.adj_batch > 0 else None
     _progress("batched adjoint leg done")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(configs.config_scene(1, args.size), args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(configs.config_scene(1, args.size), args.cpu_budget,
+                                           window_state)
         out["cpu_baseline"]["cpu_model"] = _cpu_model()
         out["cpu_baseline"]["nproc"] = os.cpu_count()
     else:
@@ -225,6 +324,7 @@ def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
     batch.sync()
     tb = time.perf_counter() - t0
     bcounts = batch.counts()
+    bwarn = batch.warnings()
     batch.close()
     singles = [ClothBody.from_scene(s) for s in scenes]
     for b in singles:
@@ -237,7 +337,9 @@ def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
     for b in singles:
         b.sync()
     tp = time.perf_counter() - t0
+    swarn = 0
     for b in singles:
+        swarn |= b.warnings()
         b.close()
     batched = ncloth * steps / tb
     plain = ncloth * steps / tp
@@ -246,7 +348,12 @@ def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
                 batched_cloth_steps_per_s=batched, plain_cloth_steps_per_s=plain,
                 batched_over_plain=batched / plain, batched_ms_per_step=tb * 1e3 / steps,
                 plain_ms_per_step=tp * 1e3 / steps, autotuned=tuned,
-                last_substep=dict(pairs=bcounts["pairs"], vf=bcounts["vf"], ee=bcounts["ee"]))
+                last_substep=dict(pairs=bcounts["pairs"], vf=bcounts["vf"], ee=bcounts["ee"]),
+                warnings=dict(batched=bwarn, plain=swarn,
+                              candidate_cap_hit=bool((bwarn | swarn) & WARN_CANDIDATE_CAP),
+                              note="CM_WARN_* bits accumulated over the run; with "
+                                   "candidate_cap_hit the two sides may skip different "
+                                   "narrow-phase work and the ratio is not comparable"))
 
 
 def large_leg(size: int, steps: int, warmup: int = 3) -> dict:

@@ -222,6 +222,8 @@ int cm_read_normals(cm_body* body, float* xyz);     /* 3N, syncs */
 #define CM_VERTEX_POSITION 0
 #define CM_VERTEX_NORMAL 7
 int cm_read_vertices(cm_body* body, float* verts);
+/* Also drops a forward recorded by cm_forward_ckpt (its cm_adjoint then returns
+ * CM_ERR_STATE). */
 int cm_set_sphere(cm_body* body, const float center[3], float radius);
 /* Replace the state. x, p, v: 3N floats each (p may be NULL: p = x);
  * tri: 4T ints (current triangle order with feature bits) or NULL. */
@@ -244,6 +246,9 @@ int cm_query_contacts(cm_body* body, cm_contact* vf, int32_t cap_vf, int32_t* n_
  * iterations on the current predicted positions (it changes them). */
 #define CM_KERNEL_SOLVE 0
 #define CM_KERNEL_SUBSTEP 1
+/* the BVH traversal of the current tree (rewrites the same per-leaf pair chunk;
+ * the compacted pair list is unchanged) */
+#define CM_KERNEL_TRAVERSE 2
 int cm_time_kernel(cm_body* body, int32_t which, int32_t reps, float* avg_ms);
 
 /* Per-stage timing of `substeps` substeps run eagerly on the body's stream with a
@@ -296,12 +301,18 @@ int cm_adjoint(cm_body* body, const float* dl_dx, const float* dl_dv, float* dl_
  * 8) and the secant pass's workgroups (1024, 2048, 4096), keeps the fastest of each,
  * and restores the state (the simulation does not advance). None of them changes a
  * result. chosen (3 ints, may be NULL) receives the picks in that order. Syncs; the
- * substep graph is captured again at the next cm_step. */
+ * substep graph is captured again at the next cm_step. The body's warnings
+ * (cm_get_warnings) keep their value from before the call; the counts and the
+ * pair / contact query buffers describe the last trial substep until the next
+ * cm_step. */
 int cm_autotune(cm_body* body, int32_t substeps, int32_t* chosen);
 
 /* ---- unit-level LBVH entry points on caller device buffers -------------- */
 /* Swept leaf AABBs (written to leaf slots [T-1, 2T-1) of d_aabb_min/max) and
- * Morton keys for triangles d_tri in their current order. */
+ * Morton keys for triangles d_tri in their current order. The values that
+ * cm_lbvh_sort permutes with the keys (the triangle slots 0..T-1) are the
+ * caller's to fill. With morton_normalize it allocates a centroid scratch of
+ * 16 (T + 2 ceil(T/256) + 2) bytes on the stream (hipMallocAsync) and frees it. */
 int cm_tri_update(int32_t n, int32_t t, const void* d_x /*float4 N*/, const void* d_p /*float4 N*/,
                   const void* d_tri /*int4 T*/, float thickness, int32_t morton_bits,
                   int32_t morton_normalize, void* d_aabb_min /*float4 2T-1*/,

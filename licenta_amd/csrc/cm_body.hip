@@ -1042,6 +1042,9 @@ int cm_set_sphere(cm_body* b, const float center[3], float radius) {
   b->prm.sphere_center[2] = center[2];
   b->prm.sphere_radius = radius;
   b->prm.flags |= CM_F_SPHERE;
+  // a recorded forward ran without this sphere: its adjoint would recompute the
+  // segments with it, so the record is dropped
+  b->adj.ready = false;
   // kernel arguments changed: re-record the substep graph
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
@@ -1174,9 +1177,15 @@ int cm_get_buffer(cm_body* b, int32_t which, void* host, int64_t capacity_bytes,
 int cm_time_kernel(cm_body* b, int32_t which, int32_t reps, float* avg_ms) {
   int rc = body_check(b);
   if (rc) return rc;
-  if (reps < 1 || !avg_ms || (which != CM_KERNEL_SOLVE && which != CM_KERNEL_SUBSTEP)) {
+  if (reps < 1 || !avg_ms ||
+      (which != CM_KERNEL_SOLVE && which != CM_KERNEL_SUBSTEP && which != CM_KERNEL_TRAVERSE)) {
     set_error("cm_time_kernel: invalid argument");
     return CM_ERR_ARG;
+  }
+  const bool coll = (b->prm.flags & CM_F_SELF_COLLISION) && b->t >= 2;
+  if (which == CM_KERNEL_TRAVERSE && !coll) {
+    set_error("cm_time_kernel: the body has no self-collision tree");
+    return CM_ERR_STATE;
   }
   if (which == CM_KERNEL_SOLVE) reps += reps & 1;  // even: the result stays in p
   rc = ensure_graph(b);
@@ -1187,8 +1196,11 @@ int cm_time_kernel(cm_body* b, int32_t which, int32_t reps, float* avg_ms) {
   CM_HIP(hipEventRecord(e0, b->stream));
   if (which == CM_KERNEL_SOLVE) {
     NarrowBufs nb = b->nb;
-    if (!((b->prm.flags & CM_F_SELF_COLLISION) && b->t >= 2)) nb.dprefix = nullptr;
+    if (!coll) nb.dprefix = nullptr;
     rc = launch_solve_only(pbd_bufs(b), pbd_params(b), nb, reps, b->stream);
+  } else if (which == CM_KERNEL_TRAVERSE) {
+    for (int r = 0; r < reps && !rc; ++r)
+      rc = launch_traverse_only(b->lb, b->cap, b->chunk, b->leaf_cnt, b->leaf_off, b->stream);
   } else {
     for (int r = 0; r < reps && !rc; ++r) {
       if (b->exec) {
@@ -1448,6 +1460,9 @@ int cm_adjoint(cm_body* b, const float* dl_dx, const float* dl_dv, float* dl_dx0
     set_error("cm_adjoint: no forward on record (cm_forward_ckpt first)");
     return CM_ERR_STATE;
   }
+  // the recompute re-runs the substep with the body's current flags: they must
+  // still describe the differentiable substep
+  if ((rc = adj_check(b, "cm_adjoint"))) return rc;
   if (!dl_dx) {
     set_error("cm_adjoint: dl_dx is required");
     return CM_ERR_ARG;
@@ -1644,6 +1659,10 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
     return code;
   };
   if (copy_state(true) != hipSuccess) return fail(CM_ERR_HIP);
+  // the trial substeps must not add to the body's accumulated warnings
+  uint32_t warn_keep[4];
+  CM_HIP(hipMemcpyAsync(warn_keep, b->warn, sizeof(warn_keep), hipMemcpyDeviceToHost, b->stream));
+  CM_HIP(hipStreamSynchronize(b->stream));
   float ms = 0.f;
   if ((rc = trial(&ms))) return fail(rc);  // warm-up
   // one parameter at a time, each over its candidates, keeping the fastest
@@ -1680,6 +1699,7 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
     b->nb.secant_grid = best;
   }
   if (copy_state(false) != hipSuccess) return fail(CM_ERR_HIP);
+  CM_HIP(hipMemcpyAsync(b->warn, warn_keep, sizeof(warn_keep), hipMemcpyHostToDevice, b->stream));
   CM_HIP(hipStreamSynchronize(b->stream));
   cleanup();
   if (chosen) {

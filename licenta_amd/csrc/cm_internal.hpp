@@ -109,6 +109,9 @@ int launch_pack_nodes(const LbvhBufs& b, hipStream_t s);
 // pairs: traversal into chunk (T*cap int2) + per-leaf counts, then compaction.
 int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
                  int2* pairs, int* d_npairs, void* scan_temp, hipStream_t s);
+// the traversal kernel alone on the current tree (it rewrites the same pair chunk)
+int launch_traverse_only(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
+                         hipStream_t s);
 int launch_gather_tris(int t, const uint32_t* perm, const int4* tri_in, int4* tri_out,
                        int fix_leaves, float4* amin, float4* amax, float4* tmp_min,
                        float4* tmp_max, hipStream_t s);

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(kBlock) k_tri_update(
     amin[t - 1 + i] = f4(smn, 0.f);
     amax[t - 1 + i] = f4(smx, 0.f);
     c = add(smn, divs(sub(smx, smn), 2.f));
-    vals[i] = (uint32_t)i;
+    if (vals) vals[i] = (uint32_t)i;
     if (centroid) {
       centroid[i] = f4(c, 0.f);
     } else if (bits == 30) {
@@ -901,6 +901,21 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   return CM_OK;
 }
 
+int launch_traverse_only(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
+                         hipStream_t s) {
+  const int t = b.t;
+  if (t < 2) return CM_OK;
+  const int nt = (t + kTravBlock - 1) / kTravBlock;
+  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 2;
+  const int round = kXcds * xchunk;
+  hipLaunchKernelGGL(k_traverse, dim3((nt + round - 1) / round * round), dim3(kTravBlock), 0, s, t,
+                     b.nodes, b.amin, b.amax,
+                     cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
+                     b.ncloth ? b.croot : nullptr, b.ccap, offsets, xchunk);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
 int launch_pack_nodes(const LbvhBufs& b, hipStream_t s) {
   const int t = b.t;
   if (t < 2) return CM_OK;
@@ -981,16 +996,13 @@ extern "C" int cm_tri_update(int32_t n, int32_t t, const void* d_x, const void* 
   const bool norm = morton_bits == 30 && morton_normalize;
   float4* cen = nullptr;
   float4* box = nullptr;
-  uint32_t* vals = nullptr;
-  CM_HIP(hipMallocAsync((void**)&vals, sizeof(uint32_t) * (size_t)t, s));
   if (norm) {
     CM_HIP(hipMallocAsync((void**)&cen, sizeof(float4) * (size_t)t, s));
     CM_HIP(hipMallocAsync((void**)&box, sizeof(float4) * (2 * (size_t)nblocks(t) + 2), s));
   }
   int rc = launch_tri_update(n, t, (const float4*)d_x, (const float4*)d_p, (const int4*)d_tri,
                              thickness, morton_bits, morton_normalize, (float4*)d_aabb_min,
-                             (float4*)d_aabb_max, d_keys, vals, cen, box, s);
-  hipFreeAsync(vals, s);
+                             (float4*)d_aabb_max, d_keys, nullptr, cen, box, s);
   if (cen) hipFreeAsync(cen, s);
   if (box) hipFreeAsync(box, s);
   return rc;

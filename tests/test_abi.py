@@ -109,3 +109,21 @@ def test_flag_and_layout_constants_match_header():
     assert _abi.VERTEX_DTYPE.itemsize == 4 * int(lay["FLOATS"]) == 48
     assert _abi.VERTEX_DTYPE.fields["position"][1] == 4 * int(lay["POSITION"])
     assert _abi.VERTEX_DTYPE.fields["normal"][1] == 4 * int(lay["NORMAL"])
+
+
+def test_integration_snippets_compile(tmp_path):
+    """The C++ blocks of INTEGRATION.md sections 2-4 (pair query on device buffers,
+    the batch, the adjoint) compile against include/clothmi.h as written."""
+    import shutil
+    import subprocess
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    body = text[text.index("## 2."):text.index("## 5.")]
+    blocks = re.findall(r"```cpp\n(.*?)```", body, flags=re.S)
+    assert len(blocks) == 3
+    src = tmp_path / "snippets.cpp"
+    src.write_text("\n".join(blocks))
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    r = subprocess.run([hipcc, "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-Wno-unused-command-line-argument",
+                        "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

@@ -190,6 +190,13 @@ def test_adjoint_rejects():
     with pytest.raises(RuntimeError, match="error -1"):
         body.set_stiffness(np.full(body.counts()["s"], 1.5, np.float32))
     body.close()
+    # a sphere set after the forward: the record is dropped, the adjoint refuses
+    body = ClothBody.from_scene(sc)
+    body.forward_ckpt(2, 1)
+    body.set_sphere_pos((0.0, -1.0, 0.0), 0.1)
+    with pytest.raises(RuntimeError, match="error -4"):
+        body.adjoint(np.zeros((body.n, 3), np.float32))
+    body.close()
     sc = dict(sc, params=dict(sc["params"], flags=sc["params"]["flags"] | configs.F_GROUND))
     body = ClothBody.from_scene(sc)
     with pytest.raises(RuntimeError, match="error -1"):

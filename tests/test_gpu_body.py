@@ -40,11 +40,24 @@ def seeded_pair(oracle_mod, scene, body):
     return out
 
 
-def compare_substep(oracle_mod, scene, body, check_tree=True, tol64=TOL64, dev=False):
+def compare_substep(oracle_mod, scene, body, check_tree=True, tol64=TOL64, dev=False,
+                    replay64=False):
+    """One substep of the GPU body against the FP32 and FP64 oracles seeded with its
+    state. With replay64 a third oracle, in FP64, takes the FP32 oracle's contact
+    activation decisions of every iteration (cc < 0, Physics/CudaPBD.cu:1295,1400)
+    instead of evaluating them: the FP64 arithmetic on the path the FP32 run took."""
     o32, o64 = seeded_pair(oracle_mod, scene, body)
+    if replay64:
+        o32.activation_mode(1)
+        o64r = seeded_pair(oracle_mod, scene, body)[1]
     body.fixed_update(1)
     o32.step(1)
     o64.step(1)
+    if replay64:
+        o64r.activation_mode(2)
+        o64r.set_activations(o32.activations())
+        o64r.step(1)
+        assert not o64r.activation_mismatch()
     t = body.t
     res = {}
     if scene["params"]["flags"] & configs.F_SELF_COLLISION:
@@ -78,6 +91,14 @@ def compare_substep(oracle_mod, scene, body, check_tree=True, tol64=TOL64, dev=F
     assert res["err64"] <= tol64, res
     if dev:  # per-vertex deviation from FP64, relative to the largest coordinate
         res["dev64"] = np.abs(np.asarray(x, np.float64) - x64).max(axis=1) / np.abs(x64).max()
+    if replay64:
+        xr, _, _ = o64r.state()
+        res["err64_replay"] = rel_err(x, xr)
+        # per-vertex relative error: |x_gpu - x_64| / |x_64| per vertex
+        d = np.linalg.norm(np.asarray(x, np.float64) - xr, axis=1)
+        res["vertex_rel64_replay"] = float((d / np.maximum(np.linalg.norm(xr, axis=1),
+                                                            1e-30)).max())
+        res["active_decisions"] = int(o32.activations().sum())
     return res
 
 
@@ -322,12 +343,13 @@ def test_full_size_sphere_drop(oracle_mod):
     First two substeps: every stage against the oracles, positions within 2e-6 of
     FP32 and 1e-5 of FP64. Substep 1,201, after the cloth has draped over the sphere
     and self-contact is at its densest (millions of pairs, tens of thousands of
-    contacts): every integer stage bit-exact and positions within 2e-6 of the FP32
-    oracle as before, but against FP64 only within 1e-4, with every vertex beyond
-    1e-5 in a contact. There a contact's inequality (distance below the thickness)
-    can switch on in one iteration in FP32 and in another in FP64 when the distance
-    sits within rounding of the thickness; the two then differ by about one contact
-    correction, which no FP32 evaluation of the algorithm avoids."""
+    contacts): every integer stage bit-exact, positions within 2e-6 of the FP32
+    oracle, and within 1e-5 relative (global and per vertex) of the FP64 oracle run
+    on the FP32 run's contact activation decisions. A contact's inequality (distance
+    below the thickness) can switch on in one iteration in FP32 and in another in
+    FP64 when the distance sits within rounding of the thickness; the FP64 oracle
+    that evaluates its own decisions then differs by about one contact correction
+    (bounded here by 1e-4, with every vertex beyond 1e-5 in a contact)."""
     from licenta_amd.body import ClothBody
     sc = configs.config_scene(1)
     body = ClothBody.from_scene(sc)
@@ -336,10 +358,11 @@ def test_full_size_sphere_drop(oracle_mod):
         compare_substep(oracle_mod, sc, body)
     body.fixed_update(1198)
     print(f"[{sc['name']}] substep 1200", flush=True)
-    r = compare_substep(oracle_mod, sc, body, tol64=1e-4, dev=True)
+    r = compare_substep(oracle_mod, sc, body, tol64=1e-4, dev=True, replay64=True)
     dev = r.pop("dev64")
     print(f"  {r}; vertices beyond 1e-5 of FP64: {int((dev > TOL64).sum())}", flush=True)
     assert r["pairs"] > 5_000_000 and r["vf"] + r["ee"] > 10_000
+    assert r["err64_replay"] <= TOL64 and r["vertex_rel64_replay"] <= TOL64
     gvf, gee = body.contacts()
     touched = np.zeros(body.n, bool)
     for c in (gvf, gee):

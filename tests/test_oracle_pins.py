@@ -384,3 +384,40 @@ def test_expandby2_multiply_mask_equals_interleave(oracle_mod):
     # planted mistake: a dropped step leaves bits unspread
     w = (q * np.uint64(0x000100000001)) & np.uint64(0xFFFF00000000FFFF)
     assert not np.array_equal(w, v)
+
+
+def test_activation_replay_mechanics(oracle_mod):
+    """The FP64 oracle replaying its own recorded contact activation decisions
+    reproduces its own run bit for bit; replaying decisions of another contact set
+    is refused (evaluated instead, with the mismatch flagged)."""
+    sc = _folded()
+    v = np.zeros((len(sc["xyz"]), 3))
+    v.reshape(16, 16, 3)[8:, :, 1] = -0.3
+    a, b = oracle_mod.OracleSim(sc, 64), oracle_mod.OracleSim(sc, 64)
+    for o in (a, b):
+        o.set_state(sc["xyz"].astype(np.float64), v)
+    a.step(1)  # the second substep has contacts active in some iterations
+    x, p, vv = a.state()
+    tri = a.get_i32("tri").reshape(-1, 4)
+    b.set_state(x, vv, p, tri)
+    a.activation_mode(1)
+    a.step(1)
+    act = a.activations()
+    assert act.sum() > 0 and len(act) == 10 * (len(a.contacts(False)) + len(a.contacts(True)))
+    b.activation_mode(2)
+    b.set_activations(act)
+    b.step(1)
+    assert not b.activation_mismatch()
+    assert all(np.array_equal(u, w) for u, w in zip(a.state(), b.state()))
+    # flipped decisions change the result; decisions of the wrong length are refused
+    b.set_state(x, vv, p, tri)
+    b.activation_mode(2)
+    b.set_activations(1 - act)
+    b.step(1)
+    assert not np.array_equal(a.state()[0], b.state()[0])
+    b.set_state(x, vv, p, tri)
+    b.activation_mode(2)
+    b.set_activations(act[:-1])
+    b.step(1)
+    assert b.activation_mismatch()
+    assert np.array_equal(a.state()[0], b.state()[0])
