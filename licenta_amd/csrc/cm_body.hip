@@ -11,6 +11,7 @@
 // host once; everything per substep runs on the body's stream.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -147,6 +148,10 @@ int dalloc(cm_body* b, T** out, size_t count) {
   }
   b->allocs.push_back(ptr);
   *out = (T*)ptr;
+  // CLOTHMI_LOG_ALLOC=1 lists the body's device allocations on stderr
+  if (std::getenv("CLOTHMI_LOG_ALLOC"))
+    std::fprintf(stderr, "clothmi alloc %zu bytes (%zu allocations)\n", sizeof(T) * count,
+                 b->allocs.size());
   return CM_OK;
 }
 
@@ -373,8 +378,10 @@ int build_topology(const cm_params& P, const float* xyz, int n, const uint32_t* 
     if (eown.emplace(edge_key(c, a), 1).second) f |= 1 << 0;
     h.tri[i] = make_int4(a, b, c, f);
   }
-  // long-range attachments (Physics/CudaPBD.cu:60-81)
-  for (int i = 0; i < n; ++i)
+  // long-range attachments (Physics/CudaPBD.cu:60-81): N x F rest lengths, built only
+  // when they are used (a batch of cloths with pinned edges would otherwise hold
+  // every vertex's distance to every pinned vertex of every cloth)
+  for (int i = 0; i < n && (P.flags & CM_F_LRA); ++i)
     if (pinned[i]) h.lra_fixed.push_back(i);
   const int F = (int)h.lra_fixed.size();
   h.lra_l0.assign((size_t)n * F, 0.f);

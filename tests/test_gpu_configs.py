@@ -1,7 +1,7 @@
 """GPU parity on the named workloads at their own sizes and budgets:
 
-* the 512-cloth batch (sides 16..128 per axis, seed 2) with its reduced candidate
-  budgets (16 VF / 24 EE candidates and 4 contacts per triangle): every cloth equal
+* the 512-cloth batch (sides 16..128 per axis, seed 2) with its candidate budgets
+  (40 VF / 96 EE candidates and 16 contacts per triangle): every cloth equal
   bit for bit to its own single body, sampled cloths against the FP32 oracle, and no
   candidate buffer overflow on either side;
 * the batched-gradient workload (64 cloths of 64 x 64, per-cloth bend stiffness,
@@ -35,7 +35,7 @@ def scaled_err(g, ref):
 def test_batch_512_own_budgets(oracle_mod):
     from licenta_amd.body import ClothBody
     scenes = configs.batched_scene(2)
-    assert len(scenes) == 512 and tuple(scenes[0]["params"]["reserved"][:3]) == (16, 24, 4)
+    assert len(scenes) == 512 and tuple(scenes[0]["params"]["reserved"][:3]) == (40, 96, 16)
     steps = 3
     print("[batch512] batched body", flush=True)
     batch = ClothBody.batch(scenes)
