@@ -897,6 +897,9 @@ struct Sim {
     std::vector<RV> x, v;
     std::vector<std::vector<RV>> pk;
     std::vector<cm_contact> vf, ee;
+    // contact activation decisions the forward took, one byte per (iteration,
+    // contact); the adjoint differentiates the substep with these decisions
+    std::vector<uint8_t> act;
   };
   bool record = false;
   std::vector<Tape> tape;
@@ -1435,10 +1438,15 @@ struct Sim {
       for (int r = 0; r < 4; ++r) rec_id[4 * c + r] = (uint32_t)k.v[r];
     }
     record_order(rec_id, order);
-    if (act_mode == 1) act.assign((size_t)prm.solver_iterations * C, 0);
     if (act_mode == 2 && act.size() != (size_t)prm.solver_iterations * C) {
       act_mismatch = true;  // decisions of another contact set: evaluate them here
       act_mode = 0;
+    }
+    // a recorded substep keeps the decisions it took (evaluated ones are recorded)
+    const int saved_mode = act_mode;
+    if (act_mode == 1 || (act_mode == 0 && record)) {
+      act.assign((size_t)prm.solver_iterations * C, 0);
+      act_mode = 1;
     }
     const int S = (int)tp.sid1.size(), B = (int)tp.bkp.size();
     std::vector<RV> slots(tp.prefix[n]);
@@ -1512,6 +1520,8 @@ struct Sim {
       for (int i = 0; i < n; ++i) p[i] = p[i] + divs(acc[i], R(cnt[i]));
     }
     if (pk_out) pk_out->push_back(p);
+    if (record) tape.back().act = act;
+    act_mode = saved_mode;
   }
 
   // ---- stage 7: integration + world clamp (Physics/CudaPBD.cu:1916-1932,
@@ -1780,8 +1790,11 @@ struct Sim {
   // VF and EE contacts: c_r = k_r g(nn), g(nn) = nn - th nn / |nn| while |nn| < th,
   // with fixed weights; nn = sum_r m_r p_r. Roles whose record is zero were neither
   // summed nor counted, so their cotangents are left out.
+  // active: the forward's activation decision for this contact and iteration (its
+  // value cc = |nn| - thickness was negative, or the decision was replayed)
   void contact_vjp(const cm_contact& k, bool isvf, const std::vector<RV>& ps,
-                   const std::vector<RV>& a, const bool used[4], std::vector<RV>& la) const {
+                   const std::vector<RV>& a, const bool used[4], bool active,
+                   std::vector<RV>& la) const {
     R m[4], kr[4];
     const int* v = k.v;
     if (isvf) {
@@ -1803,7 +1816,7 @@ struct Sim {
     RV nn = mk<R>(0, 0, 0);
     for (int r = 0; r < 4; ++r) nn = nn + mul(ps[v[r]], m[r]);
     const R l = len(nn), th = R(prm.thickness);
-    if (!(l - th < 0)) return;
+    if (!active) return;
     const RV n = divs(nn, l);
     RV G = mk<R>(0, 0, 0);
     for (int r = 0; r < 4; ++r)
@@ -1840,11 +1853,14 @@ struct Sim {
         const std::vector<RV>& ps = tpe.pk[it];
         std::vector<int> cnt(n, 0);
         for (int i = 0; i < n; ++i) cnt[i] = tp.prefix[i + 1] - tp.prefix[i];
+        const bool have_act = tpe.act.size() == (size_t)I * C;
+        std::vector<uint8_t> on(C);
         for (size_t c = 0; c < C; ++c) {
           RV cc[4];
           const bool isvf = c < tpe.vf.size();
           const cm_contact& k = isvf ? tpe.vf[c] : tpe.ee[c - tpe.vf.size()];
-          if (isvf) contact_vf(k, ps, cc); else contact_ee(k, ps, cc);
+          const int force = have_act ? (int)tpe.act[(size_t)it * C + c] : -1;
+          on[c] = isvf ? contact_vf(k, ps, cc, force) : contact_ee(k, ps, cc, force);
           for (int r = 0; r < 4; ++r) {
             used[4 * c + r] = !is_zero(cc[r]);
             cnt[k.v[r]] += used[4 * c + r];
@@ -1861,7 +1877,7 @@ struct Sim {
           const cm_contact& k = isvf ? tpe.vf[c] : tpe.ee[c - tpe.vf.size()];
           bool u[4];
           for (int r = 0; r < 4; ++r) u[r] = used[4 * c + r] != 0;
-          contact_vjp(k, isvf, ps, a, u, la);
+          contact_vjp(k, isvf, ps, a, u, on[c] != 0, la);
         }
         lp.swap(la);
       }
