@@ -489,6 +489,35 @@ int orc_record(void* hp, int on) {
   return 0;
 }
 
+// Replace the tape of dst (an FP64 handle) with the tape src (an FP32 handle)
+// recorded: the same iterates, contact sets and activation decisions, widened to
+// FP64, so dst's adjoint differentiates the FP32 forward at its own points.
+int orc_copy_tape(void* dst_p, void* src_p) {
+  Handle* dst = (Handle*)dst_p;
+  Handle* src = (Handle*)src_p;
+  if (dst->precision != 64 || src->precision != 32) return -1;
+  auto& d = *dst->d;
+  const auto& s = *src->f;
+  if (d.n != s.n) return -1;
+  auto widen = [](const std::vector<V3<float>>& a) {
+    std::vector<V3<double>> out(a.size());
+    for (size_t i = 0; i < a.size(); ++i) out[i] = cvt<double>(a[i]);
+    return out;
+  };
+  d.tape.clear();
+  for (const auto& t : s.tape) {
+    d.tape.emplace_back();
+    auto& u = d.tape.back();
+    u.x = widen(t.x);
+    u.v = widen(t.v);
+    for (const auto& pk : t.pk) u.pk.push_back(widen(pk));
+    u.vf = t.vf;
+    u.ee = t.ee;
+    u.act = t.act;
+  }
+  return 0;
+}
+
 // Adjoint of the recorded substeps. lx, lv: 3N doubles, dL/dx and dL/dv after the
 // last substep in, dL/dx and dL/dv before the first out; gk_dist (S) and gk_bend (B):
 // dL/dkp per distance / bend constraint, times dkp/dk = (1 - k)^(1/I - 1) / I when

@@ -71,6 +71,7 @@ def lib():
     L.orc_coplanarity_times.argtypes = [vp, vp, vp]
     L.orc_test_vf.argtypes = [vp, C.c_float, vp]
     L.orc_test_ee.argtypes = [vp, C.c_float, vp]
+    L.orc_copy_tape.argtypes = [vp, vp]
     L.orc_act_mode.argtypes = [vp, i32]
     L.orc_get_act.argtypes = [vp, vp, i64]
     L.orc_get_act.restype = i64
@@ -160,6 +161,12 @@ class OracleSim:
         kb = None if k_bend is None else np.ascontiguousarray(k_bend, dtype=np.float32)
         lib().orc_adjoint(self.h, _p(lx), _p(lv), _p(gs), _p(gb), _p(kd), _p(kb))
         return lx, lv, gs, gb
+
+    def copy_tape_from(self, src: "OracleSim") -> None:
+        """Take src's recorded tape (an FP32 run) as this FP64 oracle's, widened:
+        adjoint() then differentiates the FP32 forward at its own iterates."""
+        if lib().orc_copy_tape(self.h, src.h) != 0:
+            raise ValueError("copy_tape_from needs an FP64 destination and an FP32 source")
 
     def set_parents(self, parents):
         parents = np.ascontiguousarray(parents, dtype=np.int32)

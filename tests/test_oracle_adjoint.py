@@ -123,3 +123,27 @@ def test_adjoint_plants_fail(oracle_mod):
     assert abs(f - g_kd[i]) <= 1e-3 * abs(f)
     assert abs(f - (-g_kd[i])) > 0.5 * abs(f)
     assert abs(f - g_kp[i]) > 0.5 * abs(f)
+
+
+def test_adjoint_on_copied_fp32_tape(oracle_mod):
+    """The FP64 adjoint evaluated on an FP32 run's tape (copy_tape_from) agrees with
+    the FP64 run's own adjoint to the FP32 forward's rounding, and a copy into an
+    FP32 oracle is refused."""
+    from licenta_amd import configs
+    sc = configs.config_scene(3, 8)
+    rng = np.random.default_rng(4)
+    v0 = rng.normal(0.0, 0.05, (len(sc["xyz"]), 3))
+    target = sc["xyz"] + rng.normal(0.0, 0.01, (len(sc["xyz"]), 3))
+    o32, o64, o64b = (oracle_mod.OracleSim(sc, p) for p in (32, 64, 64))
+    for o in (o32, o64, o64b):
+        o.set_state(sc["xyz"].astype(np.float64), v0)
+        o.record(True)
+        o.step(3)
+    own = o64.adjoint(2.0 * (o64.state()[0] - target) / len(target))
+    o64b.copy_tape_from(o32)
+    cp = o64b.adjoint(2.0 * (o32.state()[0] - target) / len(target))
+    for a, b in zip(own, cp):
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(a).max()
+    assert not all(np.array_equal(a, b) for a, b in zip(own, cp))  # different points
+    with pytest.raises(ValueError):
+        o32.copy_tape_from(o64)
