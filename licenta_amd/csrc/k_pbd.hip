@@ -929,10 +929,12 @@ __device__ void bend_vjp(const float4 (&P)[4], const float4 (&G)[4], float kpf, 
   const V C = tadd(tcross(P2, n2), tmul(tcross(n1, P2), d));
   const V D = tadd(tcross(P2, n1), tmul(tcross(n2, P2), d));
   const T i23 = (T)1 / l23, i24 = (T)1 / l24;
+  // n1, n2 and the q_k as bend_all evaluates them: near d = +-1 the reverse sweep
+  // amplifies any difference in them
   V q[4];
-  q[2] = tmul(C, i23);
-  q[3] = tmul(D, i24);
-  q[1] = tsub(tneg(tmul(A, i23)), tmul(B, i24));
+  q[2] = bdiv<T>(C, l23);
+  q[3] = bdiv<T>(D, l24);
+  q[1] = tsub(tneg(bdiv<T>(A, l23)), bdiv<T>(B, l24));
   q[0] = tsub(tsub(tneg(q[1]), q[2]), q[3]);
   const T r = tsqrt<T>((T)1 - d * d), e = phi - phi0;
   const T term = r * e * kp;

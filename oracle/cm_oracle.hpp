@@ -1706,15 +1706,17 @@ struct Sim {
     const T l23 = len(c23), l24 = len(c24);
     const T eps = T(1e-8f);
     if (l23 <= eps || l24 <= eps) return;
-    const TV n1 = divs(c23, l23), n2 = divs(c24, l24);
+    // n1, n2 and the q_k as the forward evaluates them (bend_impl): near d = +-1 the
+    // reverse sweep amplifies any difference in them
+    const TV n1 = bdiv(c23, l23), n2 = bdiv(c24, l24);
     const T dd = dot(n1, n2);
     const T d = fclamp(dd, T(-1), T(1));
     if (d == T(-1) || d == T(1)) return;  // the branches with constant output
     const T phi = std::acos(d);
     const TV A = cross(P3, n2) + mul(cross(n1, P3), d), B = cross(P4, n1) + mul(cross(n2, P4), d);
     const TV C = cross(P2, n2) + mul(cross(n1, P2), d), D = cross(P2, n1) + mul(cross(n2, P2), d);
-    const TV q3 = divs(C, l23), q4 = divs(D, l24);
-    const TV q2 = -divs(A, l23) - divs(B, l24);
+    const TV q3 = bdiv(C, l23), q4 = bdiv(D, l24);
+    const TV q2 = -bdiv(A, l23) - bdiv(B, l24);
     const TV q1 = -q2 - q3 - q4;
     const TV q[4] = {q1, q2, q3, q4};
     const T r = std::sqrt(T(1) - d * d), e = phi - phi0;
@@ -1740,14 +1742,12 @@ struct Sim {
     qb[1] = qb[1] - qb[0];
     qb[2] = qb[2] - qb[0];
     qb[3] = qb[3] - qb[0];
-    T l23b = 0, l24b = 0;
     // q2 = -A / l23 - B / l24, q3 = C / l23, q4 = D / l24
-    const TV Ab = mul(qb[1], -T(1) / l23), Bb = mul(qb[1], -T(1) / l24);
-    l23b += dot(qb[1], A) / (l23 * l23);
-    l24b += dot(qb[1], B) / (l24 * l24);
-    const TV Cb = divs(qb[2], l23), Db = divs(qb[3], l24);
-    l23b -= dot(qb[2], C) / (l23 * l23);
-    l24b -= dot(qb[3], D) / (l24 * l24);
+    const T i23 = T(1) / l23, i24 = T(1) / l24;
+    const TV Ab = mul(qb[1], -i23), Bb = mul(qb[1], -i24);
+    const TV Cb = mul(qb[2], i23), Db = mul(qb[3], i24);
+    T l23b = (dot(qb[1], A) - dot(qb[2], C)) * i23 * i23;
+    T l24b = (dot(qb[1], B) - dot(qb[3], D)) * i24 * i24;
     TV P2b = mk<T>(0, 0, 0), P3b = mk<T>(0, 0, 0), P4b = mk<T>(0, 0, 0);
     TV n1b = mk<T>(0, 0, 0), n2b = mk<T>(0, 0, 0);
     // X = U x N + (M x U) d, for the four (U, N, M) = (P3,n2,n1), (P4,n1,n2), (P2,n2,n1), (P2,n1,n2)
@@ -1767,9 +1767,9 @@ struct Sim {
     n1b = n1b + mul(n2, db);
     n2b = n2b + mul(n1, db);
     // n = c / l, l = |c|
-    l23b -= dot(n1b, c23) / (l23 * l23);
-    l24b -= dot(n2b, c24) / (l24 * l24);
-    const TV c23b = divs(n1b, l23) + mul(n1, l23b), c24b = divs(n2b, l24) + mul(n2, l24b);
+    l23b -= dot(n1b, c23) * i23 * i23;
+    l24b -= dot(n2b, c24) * i24 * i24;
+    const TV c23b = mul(n1b, i23) + mul(n1, l23b), c24b = mul(n2b, i24) + mul(n2, l24b);
     // c23 = P2 x P3, c24 = P2 x P4
     P2b = P2b + cross(P3, c23b) + cross(P4, c24b);
     P3b = P3b + cross(c23b, P2);

@@ -6,15 +6,15 @@
   candidate buffer overflow on either side;
 * the batched-gradient workload (64 cloths of 64 x 64, per-cloth bend stiffness,
   30 substeps, checkpoints every 10): every cloth's gradients equal its single
-  body's, and the first cloth's gradients over one checkpoint segment within 1e-3
+  body's, and the first cloth's gradients over one checkpoint segment within 1e-4
   of the FP64 oracle adjoint evaluated on the FP32 forward's iterates (the FP64
   oracle's own trajectory, on the FP32 run's contact decisions, is compared and
   printed too);
 * the 128 x 128 fitting workload at the fit's start (stiffness 0.65, zero velocity,
   loss against the ground-truth run): over 10 and over 60 substeps the gradient
-  within 1e-3 of the FP64 oracle adjoint on the FP32 forward's iterates; over the
-  60 substeps (checkpoints every 10) against central finite differences (h = 1e-3)
-  of the GPU forward on 64 random coordinates;
+  within 1e-4 of the FP64 oracle adjoint on the FP32 forward's iterates; over 10
+  substeps, directional derivatives against central finite differences of the GPU
+  forward;
 * the reference's FP32 bend arithmetic (CM_F_BEND_FP64 cleared) against the FP32
   oracle.
 """
@@ -191,7 +191,7 @@ def test_batched_adjoint_64_cloths(oracle_mod):
     print(f"  cloth 0 (x, v, k_dist, k_bend; {nact} contact decisions): vs FP64 adjoint on the "
           f"FP32 iterates {['%.2e' % e for e in e_tape]}, vs the FP64 oracle's own "
           f"trajectory {['%.2e' % e for e in e_own]}", flush=True)
-    assert max(e_tape) <= 1e-3
+    assert max(e_tape) <= 1e-4
 
 
 def _fit_problem():
@@ -208,22 +208,46 @@ def _fit_problem():
     return sc, body, tri0, target, k0, c
 
 
+def _sine_bump(n_vertices):
+    """v = (0, sin(pi i / (m - 1)) sin(pi j / (m - 1)), 0) over the m x m grid."""
+    m = int(round(np.sqrt(n_vertices)))
+    t = np.sin(np.pi * np.arange(m) / (m - 1))
+    v = np.zeros((m * m, 3), np.float32)
+    v[:, 1] = np.outer(t, t).ravel()
+    return v
+
+
 @pytest.mark.timeout(900)
 def test_fit_128_gradients(oracle_mod):
     """The adjoint of the 128 x 128 fit at its start, over one checkpoint segment (10
-    substeps, where the FP32 and FP64 forward runs agree to 1e-5) and over the
-    workload's 60 substeps (checkpoints every 10): within 1e-3 of the largest
-    component of the FP64 oracle adjoint evaluated on the FP32 forward's iterates.
-    The FP64 oracle's own trajectory drifts from the FP32 one over the longer
-    horizon; its adjoint is printed beside. Over the 60 substeps also against
-    central finite differences of the GPU forward on 64 random coordinates (48
-    distance stiffnesses, 16 initial velocity components), within 2e-2 of each
-    group's largest difference."""
+    substeps, the same contact sets in FP32 and FP64) and over the workload's 60
+    substeps (checkpoints every 10): within 1e-4 of the largest component of the FP64
+    oracle adjoint evaluated on the FP32 forward's iterates. The FP64 oracle's own
+    trajectory is printed beside: it drifts from the FP32 one by 1e-4 within 10
+    substeps, because the cloth starts flat in the plane y = 0 and the bend in
+    absolute coordinates (the reference's form) then measures the angle between two
+    nearly coincident planes through the origin, which rounding moves.
+
+    Over the 10 substeps also against central differences of the GPU forward along
+    four directions: all distance stiffnesses together, random non-negative weights
+    on them, a uniform initial velocity and a smooth initial velocity field (one
+    sine bump over the grid). One coordinate at a time would not do: the loss moves
+    by about 1e-11 per unit of a single stiffness, below the FP32 forward's rounding
+    of the loss (about 1e-10); a white-noise velocity field would not either, for
+    the bend's conditioning above. Within 1e-2 of each directional derivative; the
+    white-noise field, and over the 60 substeps (where the perturbed runs also switch
+    contacts on and off) every direction, are printed."""
     from licenta_amd.body import ClothBody
     sc, body, tri0, target, k0, c = _fit_problem()
     n = c["n"]
     zero = np.zeros_like(sc["xyz"])
     kb = np.full(c["b"], sc["params"]["k_bend"], np.float32)
+    rng = np.random.default_rng(9)
+    dirs = [("k_dist, all", np.ones(c["s"], np.float32), None),
+            ("k_dist, random weights", rng.uniform(0.0, 1.0, c["s"]).astype(np.float32), None),
+            ("v0, uniform +y", None, np.tile(np.float32([0.0, 1.0, 0.0]), (n, 1))),
+            ("v0, sine bump", None, _sine_bump(n)),
+            ("v0, white noise (printed)", None, rng.normal(0.0, 1.0, (n, 3)).astype(np.float32))]
     msgs, fails = [], []
     try:
         for horizon in (10, 60):
@@ -244,41 +268,31 @@ def test_fit_128_gradients(oracle_mod):
                         f"FP64 oracle's own trajectory { {k: '%.2e' % e for k, e in eown.items()} }"
                         f", substeps with another FP64 contact set {mism}")
             print("  " + msgs[-1], flush=True)
-            if max(errs.values()) > 1e-3 or (horizon == 10 and (drift > 1e-5 or mism)):
+            if max(errs.values()) > 1e-4 or (horizon == 10 and mism):
                 fails.append(msgs[-1])
-        # central differences of the GPU forward (60 substeps) on 64 random coordinates
-        rng = np.random.default_rng(9)
-        ks = rng.choice(c["s"], 48, replace=False)
-        vs = rng.choice(3 * n, 16, replace=False)
 
-        def loss(kd, v0):
-            body.set_stiffness(kd)
-            body.set_state(sc["xyz"], v0, tri=tri0)
-            body.fixed_update(60)
-            xx = body.positions().astype(np.float64)
-            return float(((xx - target) ** 2).sum(axis=1).mean())
+            def loss(kd, v0):
+                body.set_stiffness(kd)
+                body.set_state(sc["xyz"], v0, tri=tri0)
+                body.fixed_update(horizon)
+                xx = body.positions().astype(np.float64)
+                return float(((xx - target) ** 2).sum(axis=1).mean())
 
-        h = 1e-3
-        fd, ad = [], []
-        for j in ks:
-            hi, lo = k0.copy(), k0.copy()
-            hi[j] += h
-            lo[j] -= h
-            fd.append((loss(hi, zero) - loss(lo, zero)) / (2 * h))
-            ad.append(gkd[j])
-        for j in vs:
-            hi, lo = zero.copy(), zero.copy()
-            hi.reshape(-1)[j] += h
-            lo.reshape(-1)[j] -= h
-            fd.append((loss(k0, hi) - loss(k0, lo)) / (2 * h))
-            ad.append(gv.reshape(-1)[j])
-        fd, ad = np.array(fd), np.array(ad)
-        e_k = float(np.abs(fd[:48] - ad[:48]).max() / np.abs(fd[:48]).max())
-        e_v = float(np.abs(fd[48:] - ad[48:]).max() / np.abs(fd[48:]).max())
-        msgs.append(f"60 substeps vs central differences: stiffness {e_k:.2e}, velocity {e_v:.2e}")
-        print("  " + msgs[-1], flush=True)
-        if e_k > 2e-2 or e_v > 2e-2:
-            fails.append(msgs[-1])
+            for name, uk, uv in dirs:
+                h = 1e-2 if uk is not None else 1e-3
+                if uk is not None:
+                    f = [loss(np.clip(k0 + sgn * h * uk, 0.0, 1.0), zero) for sgn in (1, -1)]
+                    ad = float(np.dot(gkd.astype(np.float64), uk))
+                else:
+                    f = [loss(k0, zero + sgn * h * uv) for sgn in (1, -1)]
+                    ad = float(np.dot(gv.astype(np.float64).ravel(), uv.ravel()))
+                fd = (f[0] - f[1]) / (2 * h)
+                e = abs(fd - ad) / max(abs(fd), 1e-30)
+                msgs.append(f"{horizon} substeps, direction {name}: adjoint {ad:.6e}, central "
+                            f"difference {fd:.6e} (h = {h}), relative {e:.2e}")
+                print("  " + msgs[-1], flush=True)
+                if horizon == 10 and e > 1e-2 and "printed" not in name:
+                    fails.append(msgs[-1])
     finally:
         body.close()
     assert not fails, fails
