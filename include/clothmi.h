@@ -317,6 +317,18 @@ int cm_tri_update(int32_t n, int32_t t, const void* d_x /*float4 N*/, const void
                   const void* d_tri /*int4 T*/, float thickness, int32_t morton_bits,
                   int32_t morton_normalize, void* d_aabb_min /*float4 2T-1*/,
                   void* d_aabb_max, uint64_t* d_keys, void* stream);
+/* cm_tri_update with no device allocation. No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The normalised 30-bit path keeps its
+ * centroid scratch in a static device array (4,227,104 bytes on each GPU the
+ * library runs on, held until the process ends), so t is at most 262,144
+ * (CM_ERR_ARG beyond). The stream must belong to the current device. Calls that
+ * overlap on one GPU share that array and race; results are those of
+ * cm_tri_update. */
+int cm_tri_update_noalloc(int32_t n, int32_t t, const void* d_x, const void* d_p,
+                          const void* d_tri, float thickness, int32_t morton_bits,
+                          int32_t morton_normalize, void* d_aabb_min, void* d_aabb_max,
+                          uint64_t* d_keys, void* stream);
 size_t cm_lbvh_sort_temp_bytes(int32_t t);
 /* Stable LSD radix sort of (d_keys, d_vals) by the low `key_bits` bits, in place. */
 int cm_lbvh_sort(int32_t t, int32_t key_bits, uint64_t* d_keys, uint32_t* d_vals, void* d_temp,

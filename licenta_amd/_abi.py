@@ -107,6 +107,7 @@ def lib():
     L.cm_query_pairs.argtypes = [vp, vp, i32, C.POINTER(i32)]
     L.cm_query_contacts.argtypes = [vp, vp, i32, C.POINTER(i32), vp, i32, C.POINTER(i32)]
     L.cm_tri_update.argtypes = [i32, i32, vp, vp, vp, C.c_float, i32, i32, vp, vp, vp, vp]
+    L.cm_tri_update_noalloc.argtypes = L.cm_tri_update.argtypes
     L.cm_lbvh_sort_temp_bytes.argtypes = [i32]
     L.cm_lbvh_sort_temp_bytes.restype = C.c_size_t
     L.cm_lbvh_sort.argtypes = [i32, i32, vp, vp, vp, C.c_size_t, vp]
@@ -139,7 +140,8 @@ EXPORTS = (
     "cm_sync",
     "cm_read_positions", "cm_read_velocities", "cm_read_normals", "cm_set_sphere",
     "cm_set_state", "cm_get_counts", "cm_get_buffer", "cm_get_warnings", "cm_query_pairs",
-    "cm_query_contacts", "cm_tri_update", "cm_lbvh_sort_temp_bytes", "cm_lbvh_sort",
+    "cm_query_contacts", "cm_tri_update", "cm_tri_update_noalloc", "cm_lbvh_sort_temp_bytes",
+    "cm_lbvh_sort",
     "cm_lbvh_build_temp_bytes", "cm_lbvh_build", "cm_lbvh_refit", "cm_lbvh_pairs_temp_bytes",
     "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times", "cm_set_stiffness", "cm_forward_ckpt",
     "cm_adjoint", "cm_autotune", "cm_read_vertices",
