@@ -112,7 +112,7 @@ struct cm_body {
   LbvhBufs lb;
   int2 *chunk = nullptr, *pairs = nullptr;
   int *leaf_cnt = nullptr, *leaf_off = nullptr;
-  int* counters = nullptr;  // [0] npairs [1] ncand_vf [2] ncand_ee [3] nvf [4] nee
+  int* counters = nullptr;  // [0] npairs [1] ncand_vf [2] ncand_ee [3] nvf [4] nee ... [8..9] traversal hand-over counts
   void* lb_temp = nullptr;  // LBVH build/refit scratch (carve_build)
   void* scan_temp = nullptr;
   NarrowBufs nb;
@@ -795,6 +795,22 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
       (rc = dalloc(b, &b->chunk, pcap)) || (rc = dalloc(b, &b->pairs, pcap)) ||
       (rc = dalloc(b, &b->leaf_cnt, t)) || (rc = dalloc(b, &b->leaf_off, t)))
     return fail(rc);
+  // two-pass traversal: leaves still walking after the budget's visits are finished
+  // by whole waves; up to a quarter of the leaves can be handed over, the rest walk on
+  // in the first pass. Off for caps above 96 (the second pass keeps at most 256 pairs
+  // per leaf). CLOTHMI_TRAV_BUDGET overrides the budget (0: one pass); the pairs are
+  // the same with any budget.
+  if ((rc = dalloc(b, &b->lb.spill, (size_t)t * kTravSpill))) return fail(rc);
+  {
+    int budget = params->pair_cap <= 96 ? 160 : 0;
+    if (const char* e = std::getenv("CLOTHMI_TRAV_BUDGET")) budget = std::max(0, std::atoi(e));
+    if (budget > 0 && t >= 2 && t < (1 << 26)) {
+      b->lb.rcap = t / 4 + 1024;
+      if ((rc = dalloc(b, &b->lb.trec, (size_t)b->lb.rcap * kTravRec))) return fail(rc);
+      b->lb.hctl = b->counters + 8;  // zeroed above
+      b->lb.trav_budget = budget;
+    }
+  }
   CM_HIP(hipMemset(b->sort_temp, 0, radix_sort_temp_bytes(t)));
   carve_build(t, b->lb_temp, &b->lb);
   CM_HIP(hipMemset(b->lb_temp, 0, lbvh_build_temp_bytes(t)));

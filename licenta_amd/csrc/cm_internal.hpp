@@ -71,6 +71,11 @@ int fill_i32(int* p, int v, int64_t n, hipStream_t s);
 int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 
 // ---- LBVH (k_lbvh.hip) -----------------------------------------------------
+// ints per traversal hand-over record (leaf, pairs, depth, node, cap, visits, stack)
+constexpr int kTravRec = 72;
+// ints per leaf for the traversal stack entries below its LDS part
+constexpr int kTravSpill = 47;
+
 struct LbvhBufs {
   int t = 0;
   float4 *amin = nullptr, *amax = nullptr;  // 2T-1 (internal nodes first)
@@ -83,6 +88,15 @@ struct LbvhBufs {
   int *trm = nullptr, *tdone = nullptr;
   float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
   int trav_chunk = 2;       // traversal: tiles per chunk dealt to one XCD (launch shape only)
+  // traversal: visits a leaf makes in the first pass before a wave takes it over (0:
+  // one pass); records of kRec ints for up to rcap such leaves; hctl[0..1] their
+  // count and the second pass's finished-workgroup count, zero between launches.
+  // The pair lists are the same with any budget.
+  int trav_budget = 0;
+  int* trec = nullptr;
+  int rcap = 0;
+  int* hctl = nullptr;
+  int* spill = nullptr;  // kTravSpill ints per leaf: deep traversal stack entries
   uint32_t* warn = nullptr;
   // batch of cloths (ncloth == 0: one cloth): cloth c owns triangle slots (and, after
   // the sort, leaves) [tstart[c], tstart[c + 1]); its Morton keys carry c above bit 30

@@ -17,6 +17,9 @@
 // children.
 #include "cm_internal.hpp"
 
+#include <algorithm>
+#include <climits>
+
 namespace cm {
 
 namespace {
@@ -603,41 +606,48 @@ __global__ void __launch_bounds__(1024) k_refit_top(
 }
 
 // ---- traversal ----------------------------------------------------------------
-constexpr int kTravBlock = 128;  // 2 waves; the 63-entry stacks take 31.5 KiB of LDS
+constexpr int kTravBlock = 128;  // 2 waves
 constexpr int kStack = 63;       // usable entries, as the reference's int[64] with its -1 sentinel
+// The first kLdsStack entries of a lane's stack live in LDS (8 KiB per workgroup, so
+// LDS leaves room for 8 waves per SIMD); deeper entries, rare, go to the lane's
+// global spill row of kStack - kLdsStack ints.
+constexpr int kLdsStack = 16;
+constexpr int kSpill = kStack - kLdsStack;
+static_assert(kSpill == kTravSpill, "spill row");
 
 __device__ __forceinline__ void put_pair(int2* p, int a, int b) {
-  // streamed past the caches so the node records stay resident in L2
-  __builtin_nontemporal_store((long long)(uint32_t)a | ((long long)b << 32), (long long*)p);
+  // a plain store: a leaf's chunk fills whole lines in L2 before they go out (streamed
+  // stores left each 8-byte pair a partial-line write; the 512-cloth batch traversal
+  // took 2.2 ms with them, 1.36 ms without)
+  *(long long*)p = (long long)(uint32_t)a | ((long long)b << 32);
 }
 
-// DFS of one leaf (Collision/CudaBVH.cu:23-113): its pairs in the reference's order
-// into out, at most cap of them; returns their number.
-__device__ __forceinline__ int leaf_pairs(int t, int leaf, const float4* __restrict__ nodes,
-                                          const float4* __restrict__ amin,
-                                          const float4* __restrict__ amax, int cap,
-                                          int2* __restrict__ chunk, uint32_t* __restrict__ warn,
-                                          const int* __restrict__ slot_cloth,
-                                          const int* __restrict__ croot,
-                                          const int* __restrict__ ccap,
-                                          int (*sstk)[kTravBlock]) {
-  const int nin = t - 1;
-  const int me = nin + leaf;
-  const f3 mn = xyz(amin[me]), mx = xyz(amax[me]);
-  int2* out = chunk + (int64_t)leaf * cap;
-  int node = 0;  // root (a batch: the root of the leaf's cloth, with the cloth's cap)
-  if (croot) {
-    const int c = slot_cloth[leaf];
-    node = croot[c];
-    cap = ccap[c];
-    if (node < 0) return 0;  // a one-triangle cloth has no pairs
-  }
-  int ncol = 0;
-  uint32_t w = 0;
-  int top = 0;
-  const long long limit = 64ll * t + 4096;
-  long long visits = 0;
+// State of one leaf's DFS between two visits: the node to visit next, the stack
+// depth, the pairs written and the visits made.
+struct DfsState {
+  int node, top, ncol;
+  long long visits;
+};
+
+// DFS of one leaf (Collision/CudaBVH.cu:23-113) from state st: its pairs in the
+// reference's order into out (at most cap), stack entry k in sstk[k][col]. Returns
+// true when the walk ended (stack empty, the first dropped pair, or the visit limit);
+// false when `budget` visits had been made first, with st at the next visit.
+// The stack's first kLds entries are sstk[k][col], the rest spill[k - kLds].
+template <int kRow, int kLds>
+__device__ __forceinline__ bool dfs_run(int nin, int me, int leaf, f3 mn, f3 mx,
+                                        const float4* __restrict__ nodes, int cap,
+                                        int2* __restrict__ out, long long limit,
+                                        long long budget, DfsState& st, uint32_t& w,
+                                        int (*sstk)[kRow], int col, int* __restrict__ spill) {
+  int node = st.node, top = st.top, ncol = st.ncol;
+  long long visits = st.visits;
+  bool done = true;
   while (true) {
+    if (visits >= budget) {
+      done = false;
+      break;
+    }
     if (++visits > limit) {
       w |= CM_WARN_TRAVERSAL_LIMIT;
       break;
@@ -667,18 +677,41 @@ __device__ __forceinline__ int leaf_pairs(int t, int leaf, const float4* __restr
     if (tl | tr) {
       node = tl ? l : r;
       if (tl & tr) {
-        if (top < kStack) sstk[top++][threadIdx.x] = r;
-        else w |= CM_WARN_STACK_OVERFLOW;  // the push is dropped, as in the reference
+        if (top < kStack) {
+          if constexpr (kLds < kStack) {
+            if (top < kLds) sstk[top][col] = r;
+            else spill[top - kLds] = r;
+          } else {
+            sstk[top][col] = r;
+          }
+          ++top;
+        } else {
+          w |= CM_WARN_STACK_OVERFLOW;  // the push is dropped, as in the reference
+        }
       }
     } else {
       if (top == 0) break;
-      node = sstk[--top][threadIdx.x];
+      --top;
+      if constexpr (kLds < kStack) node = top < kLds ? sstk[top][col] : spill[top - kLds];
+      else node = sstk[top][col];
     }
   }
-  if (w) atomicOr(warn, w);
-  return ncol;
+  st.node = node;
+  st.top = top;
+  st.ncol = ncol;
+  st.visits = visits;
+  return done;
 }
 
+// A leaf whose walk outlasts the first pass's visit budget leaves its state in a
+// record of kRec ints: leaf, pairs written, stack depth, next node, cap, visits, then
+// the kStack stack entries. The second pass finishes it with a whole wave.
+constexpr int kRec = kTravRec;
+static_assert(kRec >= 6 + kStack, "record holds the stack");
+
+// First pass: one lane per leaf, at most `budget` visits (no budget: budget <= 0 or
+// no record buffer). A leaf still walking then takes a record slot, if one is left,
+// and counts zero pairs here; past the slots it walks on to the end.
 __global__ void __launch_bounds__(kTravBlock) k_traverse(int t, const float4* __restrict__ nodes,
                                                          const float4* __restrict__ amin,
                                                          const float4* __restrict__ amax, int cap,
@@ -689,10 +722,13 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int t, const float4* __
                                                          const int* __restrict__ croot,
                                                          const int* __restrict__ ccap,
                                                          int* __restrict__ tile_sum,
-                                                         int xchunk) {
+                                                         int xchunk, int budget,
+                                                         int* __restrict__ rec, int rcap,
+                                                         int* __restrict__ hctl,
+                                                         int* __restrict__ spill) {
   // Entry k of every lane's stack is one row, so a wave's pushes and pops at equal
   // depths hit 64 consecutive banks.
-  __shared__ int sstk[kStack][kTravBlock];
+  __shared__ int sstk[kLdsStack][kTravBlock];
   // Workgroups are dealt to the 8 XCDs in turn. Each XCD takes every eighth chunk of
   // `xchunk` consecutive tiles (Morton-ordered leaves), so its L2 serves compact
   // subtrees while the dense regions of the cloth, whose leaves visit many more
@@ -701,10 +737,66 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int t, const float4* __
   const int tile = ((j / xchunk) * kXcds + xcd) * xchunk + j % xchunk;
   if (tile >= (t + kTravBlock - 1) / kTravBlock) return;
   const int leaf = tile * kTravBlock + threadIdx.x;
-  const int ncol = leaf < t ? leaf_pairs(t, leaf, nodes, amin, amax, cap, chunk, warn, slot_cloth,
-                                         croot, ccap, sstk)
-                            : 0;
+  int ncol = 0;
+  const int stride = cap;  // every leaf's chunk holds the body's cap (a cloth's may be less)
+  uint32_t w = 0;
+  const int nin = t - 1;
+  const int me = nin + leaf;
+  DfsState st{0, 0, 0, 0};  // root (a batch: the root of the leaf's cloth, its cap)
+  bool walk = leaf < t;
+  if (walk && croot) {
+    const int c = slot_cloth[leaf];
+    st.node = croot[c];
+    cap = ccap[c];
+    walk = st.node >= 0;  // a one-triangle cloth has no pairs
+  }
+  f3 mn = mk3(0, 0, 0), mx = mk3(0, 0, 0);
+  int2* out = chunk + (int64_t)leaf * stride;
+  int* sp = spill + (int64_t)leaf * kSpill;
+  const long long limit = 64ll * t + 4096;
+  bool done = true;
+  if (walk) {
+    mn = xyz(amin[me]);
+    mx = xyz(amax[me]);
+    const bool budgeted = budget > 0 && rec;
+    done = dfs_run<kTravBlock, kLdsStack>(nin, me, leaf, mn, mx, nodes, cap, out, limit,
+                                          budgeted ? (long long)budget : LLONG_MAX, st, w, sstk,
+                                          threadIdx.x, sp);
+  }
+  // Leaves still walking claim record slots with one atomic per wave (thousands of
+  // lanes on one counter would queue at its L2 channel).
+  const int lane = threadIdx.x & 63;
+  const uint64_t want = __ballot(!done);
+  bool handed = false;
+  if (want) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(hctl, __popcll(want));
+    base = __shfl(base, 0, 64);
+    if (!done) {
+      const int slot = base + __popcll(want & ((1ull << lane) - 1));
+      if (slot < rcap) {
+        int* R = rec + (int64_t)slot * kRec;
+        R[0] = leaf;
+        R[1] = st.ncol;
+        R[2] = st.top;
+        R[3] = st.node;
+        R[4] = cap;
+        R[5] = (int)st.visits;
+        for (int k = 0; k < st.top; ++k)
+          R[6 + k] = k < kLdsStack ? sstk[k][threadIdx.x] : sp[k - kLdsStack];
+        handed = true;
+      } else {
+        dfs_run<kTravBlock, kLdsStack>(nin, me, leaf, mn, mx, nodes, cap, out, limit, LLONG_MAX,
+                                       st, w, sstk, threadIdx.x, sp);
+      }
+    }
+  }
+  if (walk) ncol = handed ? 0 : st.ncol;
   if (leaf < t) counts[leaf] = ncol;
+  // one warning atomic per wave: two thirds of the leaves of a draped cloth hit the cap
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) w |= __shfl_xor(w, o, 64);
+  if (lane == 0 && w) atomicOr(warn, w);
   // pairs of the tile (its leaves' counts follow in the compaction)
   __shared__ int wsum[kTravBlock / 64];
   int v = ncol;
@@ -714,11 +806,252 @@ __global__ void __launch_bounds__(kTravBlock) k_traverse(int t, const float4* __
   __syncthreads();
   if (threadIdx.x == 0) {
     int tot = 0;
-    for (int w = 0; w < kTravBlock / 64; ++w) tot += wsum[w];
+    for (int q = 0; q < kTravBlock / 64; ++q) tot += wsum[q];
     tile_sum[tile] = tot;
   }
 }
 
+// ---- second pass: one wave finishes one handed-over leaf -------------------------
+// The reference's DFS visits nodes in preorder and emits a node's left leaf child
+// before its right one, so a pair's place in the leaf's list is the preorder place of
+// the node that emits it. The wave expands up to 64 pending nodes at once, in any
+// order, and tags every pair with a 64-bit key that sorts in that preorder:
+//   bits 63..58  rank of the pending subtree handed over (0 = the next node to visit,
+//                then the stack entries from the top down: they are visited in that
+//                order, each subtree before the next)
+//   bits 57..8   the path below that subtree's root, one bit per level (0 = left)
+//   bits  7..1   the depth of that path
+//   bit   0      0 for the left leaf child, 1 for the right one
+// Sorting (rank, path padded with zeros, depth, side) is preorder: a node comes before
+// its descendants (same padded path, smaller depth) and a left subtree before the
+// right one. Pending nodes are kept sorted by key (the smallest on top), so every pair
+// below the top pending node's key is final. The wave stops once more than the
+// remaining cap of pairs are final: the first `cap` are the leaf's list and one more
+// means the reference dropped pairs (CM_WARN_PAIR_CAP). A push the reference would
+// drop (stack full) is mirrored by tracking each pending node's stack depth: a node
+// whose children are both walked at depth kStack loses its right child, and
+// CM_WARN_STACK_OVERFLOW is raised when such a node precedes the point where the
+// reference's walk stopped. A leaf that overflows this pass's buffers or path bits,
+// or could reach the visit limit, is walked by lane 0 alone from its record.
+constexpr int kPend = 512;     // pending nodes per leaf
+constexpr int kHits = 256;     // pairs kept per leaf
+constexpr int kPathBits = 50;
+constexpr int kResumeBlocks = 4096;
+constexpr int kResumeMaxCap = 96;  // kHits holds cap + 1 kept pairs and a round's 128 new ones
+static_assert(kResumeMaxCap + 1 + 128 <= kHits, "hit buffer");
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int& total) {
+  int s = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(s, o, 64);
+    if (lane >= o) s += y;
+  }
+  total = __shfl(s, 63, 64);
+  return s - v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64) k_traverse_resume(int t, const float4* __restrict__ nodes,
+                                                        const float4* __restrict__ amin,
+                                                        const float4* __restrict__ amax,
+                                                        int2* __restrict__ chunk, int stride,
+                                                        int* __restrict__ counts,
+                                                        uint32_t* __restrict__ warn,
+                                                        int* __restrict__ tile_sum,
+                                                        const int* __restrict__ rec, int rcap,
+                                                        int* __restrict__ hctl) {
+  __shared__ int pe[kPend];  // node | stack depth << 26
+  __shared__ uint64_t pk[kPend];
+  __shared__ uint64_t hk[kHits];
+  __shared__ int hp[kHits];
+  __shared__ int fstk[kStack][1];
+  __shared__ unsigned long long s_ovf, s_stop;
+  const int lane = threadIdx.x;
+  const int nin = t - 1;
+  const long long limit = 64ll * t + 4096;
+  const int nres = min(hctl[0], rcap);
+  uint32_t wacc = 0;
+  for (int rid = blockIdx.x; rid < nres; rid += gridDim.x) {
+    const int* R = rec + (int64_t)rid * kRec;
+    const int leaf = R[0], ncol0 = R[1], top0 = R[2], cur = R[3], cap = R[4];
+    const long long visits0 = R[5];
+    const int me = nin + leaf;
+    const f3 mn = xyz(amin[me]), mx = xyz(amax[me]);
+    const int capr = cap - ncol0;  // pairs still to find (0: only whether one more exists)
+    int2* out = chunk + (int64_t)leaf * stride;
+    // pending: the stack entries bottom-up, then the next node on top
+    const int m = top0 + 1;
+    for (int k = lane; k < m; k += 64) {
+      pe[k] = (k < top0 ? R[6 + k] : cur) | (k << 26);  // visited at stack depth k
+      pk[k] = (uint64_t)(top0 - k) << 58;
+    }
+    if (lane == 0) {
+      s_ovf = ~0ull;
+      s_stop = ~0ull;
+    }
+    __syncthreads();
+    int np = m, nh = 0;
+    long long expanded = 0;
+    bool fallback = cap > kResumeMaxCap;
+    while (np > 0 && !fallback) {
+      const int take = min(64, np);
+      const int base = np - take;
+      const bool act = lane < take;
+      int node = 0, sd = 0;
+      uint64_t key = 0;
+      if (act) {
+        const int e = pe[np - 1 - lane];
+        node = e & 0x3ffffff;
+        sd = e >> 26;
+        key = pk[np - 1 - lane];
+      }
+      __syncthreads();
+      expanded += take;
+      int nc = 0, ne = 0, ce[2] = {0, 0}, ep[2] = {0, 0};
+      uint64_t ck[2] = {0, 0}, ek[2] = {0, 0};
+      bool deep = false;
+      if (act) {
+        const float4* nd = nodes + 4 * (int64_t)node;
+        const float4 r0 = nd[0], r1 = nd[1], r2 = nd[2], r3 = nd[3];
+        const int l = __float_as_int(r0.x), r = __float_as_int(r0.y);
+        const int rml = __float_as_int(r0.z), rmr = __float_as_int(r0.w);
+        const bool ol = (rml > me) & overlap(mn, mx, mk3(r1.x, r1.y, r1.z), mk3(r1.w, r2.x, r2.y));
+        const bool orr =
+            (rmr > me) & overlap(mn, mx, mk3(r2.z, r2.w, r3.x), mk3(r3.y, r3.z, r3.w));
+        if (ol & (l >= nin)) {
+          ek[ne] = key;
+          ep[ne++] = l - nin;
+        }
+        if (orr & (r >= nin)) {
+          ek[ne] = key | 1ull;
+          ep[ne++] = r - nin;
+        }
+        const bool tl = ol & (l < nin), tr = orr & (r < nin);
+        if (tl | tr) {
+          const int d = (int)((key >> 1) & 127ull);
+          if (d + 1 > kPathBits) deep = true;
+          const uint64_t kl = (key & ~0xffull) | ((uint64_t)(d + 1) << 1);
+          const uint64_t kr = kl | (1ull << (57 - d));
+          const bool both = tl & tr;
+          if (tl) {
+            ce[nc] = l | ((both && sd < kStack ? sd + 1 : sd) << 26);
+            ck[nc++] = kl;
+          }
+          if (tr) {
+            if (both && sd >= kStack) {
+              atomicMin(&s_ovf, (unsigned long long)key);
+            } else {
+              ce[nc] = r | (sd << 26);
+              ck[nc++] = kr;
+            }
+          }
+        }
+      }
+      int totc, tote;
+      const int exc = wave_excl_scan(nc, lane, totc);
+      const int exe = wave_excl_scan(ne, lane, tote);
+      if (__any(deep) || base + totc > kPend || visits0 + expanded > limit) {
+        fallback = true;
+        break;
+      }
+      if (nh + tote > kHits) {
+        // keep the capr + 1 smallest keys: a larger one can only move further back
+        // once the pending subtrees add theirs, so it is never listed nor the stop
+        int rk[kHits / 64];
+        uint64_t kk[kHits / 64];
+        int kp[kHits / 64];
+#pragma unroll
+        for (int u = 0; u < kHits / 64; ++u) {
+          const int i = lane + 64 * u;
+          rk[u] = kHits;
+          if (i < nh) {
+            kk[u] = hk[i];
+            kp[u] = hp[i];
+            int rank = 0;
+            for (int j2 = 0; j2 < nh; ++j2) rank += hk[j2] < kk[u];
+            rk[u] = rank;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kHits / 64; ++u)
+          if (rk[u] <= capr) {
+            hk[rk[u]] = kk[u];
+            hp[rk[u]] = kp[u];
+          }
+        nh = min(nh, capr + 1);
+        __syncthreads();
+      }
+      for (int q = 0; q < nc; ++q) {
+        const int at = base + totc - 1 - (exc + q);
+        pe[at] = ce[q];
+        pk[at] = ck[q];
+      }
+      for (int q = 0; q < ne; ++q) {
+        hk[nh + exe + q] = ek[q];
+        hp[nh + exe + q] = ep[q];
+      }
+      __syncthreads();
+      np = base + totc;
+      nh += tote;
+      if (nh > capr) {
+        const uint64_t bound = np > 0 ? pk[np - 1] : ~0ull;
+        int below = 0;
+        for (int i = lane; i < nh; i += 64) below += hk[i] < bound;
+        if (wave_sum(below) > capr) break;
+      }
+    }
+    int nout;
+    uint32_t w = 0;
+    if (fallback) {
+      __syncthreads();
+      nout = 0;
+      if (lane == 0) {
+        for (int k = 0; k < top0; ++k) fstk[k][0] = R[6 + k];
+        DfsState st{cur, top0, ncol0, visits0};
+        dfs_run<1, kStack>(nin, me, leaf, mn, mx, nodes, cap, out, limit, LLONG_MAX, st, w, fstk,
+                           0, nullptr);
+        nout = st.ncol;
+      }
+    } else {
+      // each pair's place is the number of smaller keys (keys are distinct)
+      for (int i = lane; i < nh; i += 64) {
+        const uint64_t ki = hk[i];
+        int rank = 0;
+        for (int j2 = 0; j2 < nh; ++j2) rank += hk[j2] < ki;
+        if (rank < capr) put_pair(out + ncol0 + rank, leaf, hp[i]);
+        if (rank == capr) s_stop = ki & ~1ull;  // the node where the reference stopped
+      }
+      __syncthreads();
+      nout = ncol0 + min(nh, capr);
+      if (nh > capr) w |= CM_WARN_PAIR_CAP;
+      if (s_ovf != ~0ull && s_ovf < s_stop) w |= CM_WARN_STACK_OVERFLOW;
+    }
+    if (lane == 0) {
+      counts[leaf] = nout;
+      atomicAdd(&tile_sum[leaf / kTravBlock], nout);
+      wacc |= w;
+    }
+    __syncthreads();
+  }
+  // the workgroup's warnings in one atomic; the last workgroup to finish clears the
+  // record count for the next launch
+  if (lane == 0) {
+    if (wacc) atomicOr(warn, wacc);
+    __threadfence();
+    if (atomicAdd(&hctl[1], 1) == (int)gridDim.x - 1) {
+      hctl[0] = 0;
+      hctl[1] = 0;
+      __threadfence();
+    }
+  }
+}
 
 // packed node records from separate arrays (unit-level pair query)
 __global__ void __launch_bounds__(kBlock) k_pack_nodes(int t, const int* __restrict__ lefts,
@@ -878,6 +1211,27 @@ int launch_refit(const LbvhBufs& b, hipStream_t s) {
   return CM_OK;
 }
 
+// both traversal passes (the second only with a budget and record buffers)
+static int launch_traverse(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
+                           hipStream_t s) {
+  const int t = b.t;
+  const int nt = (t + kTravBlock - 1) / kTravBlock;
+  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 2;
+  const int round = kXcds * xchunk;
+  const bool two = b.trav_budget > 0 && b.trec && b.hctl && b.rcap > 0;
+  hipLaunchKernelGGL(k_traverse, dim3((nt + round - 1) / round * round), dim3(kTravBlock), 0, s, t,
+                     b.nodes, b.amin, b.amax,
+                     cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
+                     b.ncloth ? b.croot : nullptr, b.ccap, offsets, xchunk,
+                     two ? b.trav_budget : 0, two ? b.trec : nullptr, b.rcap, b.hctl, b.spill);
+  if (two)
+    hipLaunchKernelGGL(k_traverse_resume, dim3(std::min(b.rcap, kResumeBlocks)), dim3(64), 0, s,
+                       t, b.nodes, b.amin, b.amax, chunk, cap, counts, b.warn, offsets, b.trec,
+                       b.rcap, b.hctl);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
 int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets, int2* pairs,
                  int* d_npairs, void* scan_temp, hipStream_t s) {
   const int t = b.t;
@@ -885,15 +1239,10 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   // offsets[0, nt): pair count of each tile of kTravBlock leaves; offsets[nt, 2nt): the
   // tiles' exclusive prefixes (2 * nt <= t)
   const int nt = (t + kTravBlock - 1) / kTravBlock;
-  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 2;
-  const int round = kXcds * xchunk;
-  hipLaunchKernelGGL(k_traverse, dim3((nt + round - 1) / round * round), dim3(kTravBlock), 0, s, t,
-                     b.nodes, b.amin, b.amax,
-                     cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
-                     b.ncloth ? b.croot : nullptr, b.ccap, offsets, xchunk);
-  CM_LAUNCH_CHECK();
-  int rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
-                      nullptr, s);
+  int rc = launch_traverse(b, cap, chunk, counts, offsets, s);
+  if (rc) return rc;
+  rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
+                  nullptr, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_compact_pairs, dim3(nt), dim3(kBlock), 0, s, t, cap, chunk, counts,
                      offsets + nt, pairs);
@@ -903,17 +1252,8 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
 
 int launch_traverse_only(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
                          hipStream_t s) {
-  const int t = b.t;
-  if (t < 2) return CM_OK;
-  const int nt = (t + kTravBlock - 1) / kTravBlock;
-  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 2;
-  const int round = kXcds * xchunk;
-  hipLaunchKernelGGL(k_traverse, dim3((nt + round - 1) / round * round), dim3(kTravBlock), 0, s, t,
-                     b.nodes, b.amin, b.amax,
-                     cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
-                     b.ncloth ? b.croot : nullptr, b.ccap, offsets, xchunk);
-  CM_LAUNCH_CHECK();
-  return CM_OK;
+  if (b.t < 2) return CM_OK;
+  return launch_traverse(b, cap, chunk, counts, offsets, s);
 }
 
 int launch_pack_nodes(const LbvhBufs& b, hipStream_t s) {
@@ -975,7 +1315,8 @@ size_t carve_build(int t, void* d_temp, LbvhBufs* b) {
 size_t lbvh_build_temp_bytes(int t) { return carve_build(t, nullptr, nullptr); }
 size_t lbvh_pairs_temp_bytes(int t, int cap) {
   return align_up(sizeof(int2) * (size_t)t * cap) + 2 * align_up(sizeof(int) * (size_t)t) +
-         align_up(4 * sizeof(float4) * (size_t)t) + scan_temp_bytes(t);
+         align_up(4 * sizeof(float4) * (size_t)t) + align_up(sizeof(int) * kSpill * (size_t)t) +
+         scan_temp_bytes(t);
 }
 
 }  // namespace cm
@@ -1103,6 +1444,8 @@ extern "C" int cm_lbvh_pairs(int32_t t, const int32_t* d_lefts, const int32_t* d
   tp += align_up(sizeof(int) * (size_t)t);
   b.nodes = (float4*)tp;
   tp += align_up(4 * sizeof(float4) * (size_t)t);
+  b.spill = (int*)tp;
+  tp += align_up(sizeof(int) * kSpill * (size_t)t);
   int rc = launch_pack_nodes(b, (hipStream_t)stream);
   if (rc) return rc;
   rc = scan_temp_clear(tp, t, (hipStream_t)stream);
