@@ -112,7 +112,7 @@ struct cm_body {
   LbvhBufs lb;
   int2 *chunk = nullptr, *pairs = nullptr;
   int *leaf_cnt = nullptr, *leaf_off = nullptr;
-  int* counters = nullptr;  // [0] npairs [1] ncand_vf [2] ncand_ee [3] nvf [4] nee ... [8..9] traversal hand-over counts
+  int* counters = nullptr;  // [0] npairs [1] ncand_vf [2] ncand_ee [3] nvf [4] nee ... [8] traversal hand-over count
   void* lb_temp = nullptr;  // LBVH build/refit scratch (carve_build)
   void* scan_temp = nullptr;
   NarrowBufs nb;

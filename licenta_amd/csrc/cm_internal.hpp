@@ -89,9 +89,9 @@ struct LbvhBufs {
   float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
   int trav_chunk = 2;       // traversal: tiles per chunk dealt to one XCD (launch shape only)
   // traversal: visits a leaf makes in the first pass before a wave takes it over (0:
-  // one pass); records of kRec ints for up to rcap such leaves; hctl[0..1] their
-  // count and the second pass's finished-workgroup count, zero between launches.
-  // The pair lists are the same with any budget.
+  // one pass); records of kTravRec ints for up to rcap such leaves; hctl[0] their
+  // count, zero between substeps (the pair compaction clears it). The pair lists are
+  // the same with any budget.
   int trav_budget = 0;
   int* trec = nullptr;
   int rcap = 0;

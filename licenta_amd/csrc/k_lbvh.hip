@@ -1040,17 +1040,13 @@ __global__ void __launch_bounds__(64) k_traverse_resume(int t, const float4* __r
     }
     __syncthreads();
   }
-  // the workgroup's warnings in one atomic; the last workgroup to finish clears the
-  // record count for the next launch
-  if (lane == 0) {
-    if (wacc) atomicOr(warn, wacc);
-    __threadfence();
-    if (atomicAdd(&hctl[1], 1) == (int)gridDim.x - 1) {
-      hctl[0] = 0;
-      hctl[1] = 0;
-      __threadfence();
-    }
-  }
+  // the workgroup's warnings in one atomic (the record count is cleared by the next
+  // kernel on the stream, once every workgroup here has read it)
+  if (lane == 0 && wacc) atomicOr(warn, wacc);
+}
+
+__global__ void k_clear_records(int* __restrict__ hctl) {
+  if (threadIdx.x == 0) hctl[0] = 0;
 }
 
 // packed node records from separate arrays (unit-level pair query)
@@ -1079,8 +1075,12 @@ __global__ void __launch_bounds__(kBlock) k_compact_pairs(int t, int cap,
                                                           const int2* __restrict__ chunk,
                                                           const int* __restrict__ counts,
                                                           const int* __restrict__ tile_pre,
-                                                          int2* __restrict__ pairs) {
+                                                          int2* __restrict__ pairs,
+                                                          int* __restrict__ hctl) {
   static_assert(kTravBlock == 128, "two leaves per lane of the first wave");
+  // the traversal's hand-over record count, read by its second pass, starts the next
+  // substep at zero
+  if (hctl && blockIdx.x == 0 && threadIdx.x == 0) hctl[0] = 0;
   __shared__ int pre[kTravBlock + 1];
   const int tile = blockIdx.x, tid = threadIdx.x;
   const int l0 = tile * kTravBlock;
@@ -1244,8 +1244,9 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
                   nullptr, s);
   if (rc) return rc;
+  const bool two = b.trav_budget > 0 && b.trec && b.hctl && b.rcap > 0;
   hipLaunchKernelGGL(k_compact_pairs, dim3(nt), dim3(kBlock), 0, s, t, cap, chunk, counts,
-                     offsets + nt, pairs);
+                     offsets + nt, pairs, two ? b.hctl : nullptr);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
@@ -1253,7 +1254,11 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
 int launch_traverse_only(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offsets,
                          hipStream_t s) {
   if (b.t < 2) return CM_OK;
-  return launch_traverse(b, cap, chunk, counts, offsets, s);
+  const int rc = launch_traverse(b, cap, chunk, counts, offsets, s);
+  if (rc || !(b.trav_budget > 0 && b.trec && b.hctl && b.rcap > 0)) return rc;
+  hipLaunchKernelGGL(k_clear_records, dim3(1), dim3(64), 0, s, b.hctl);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
 }
 
 int launch_pack_nodes(const LbvhBufs& b, hipStream_t s) {
