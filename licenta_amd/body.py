@@ -192,7 +192,7 @@ class ClothBody:
         out = np.zeros(3, dtype=np.int32)
         check(lib().cm_autotune(self.h, int(substeps), ptr(out)))
         return dict(tile_vertices=int(out[0]), traversal_chunk=int(out[1]),
-                    secant_workgroups=int(out[2]))
+                    traversal_budget=int(out[2]))
 
     # ---- differentiable rollout ------------------------------------------------------
     def set_stiffness(self, k_dist=None, k_bend=None) -> None:

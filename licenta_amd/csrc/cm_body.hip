@@ -122,7 +122,7 @@ struct cm_body {
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
   // launch shapes (cm_autotune): Jacobi tile vertices; traversal tiles per XCD chunk
-  // and the secant pass's workgroups live in lb / nb
+  // and the traversal's visit budget live in lb
   int tile_v = kTileV;
   // host copies
   std::vector<float> inv_mass_h;
@@ -1710,16 +1710,16 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
     }
     b->lb.trav_chunk = best;
   }
-  {
-    const int cand[3] = {1024, 2048, 4096};
-    int best = b->nb.secant_grid;
+  if (b->lb.trav_budget > 0) {  // the two-pass traversal is on
+    const int cand[3] = {128, 160, 240};
+    int best = b->lb.trav_budget;
     float best_ms = 1e30f;
     for (int c : cand) {
-      b->nb.secant_grid = c;
+      b->lb.trav_budget = c;
       if ((rc = trial(&ms))) return fail(rc);
       if (ms < best_ms) { best_ms = ms; best = c; }
     }
-    b->nb.secant_grid = best;
+    b->lb.trav_budget = best;
   }
   if (copy_state(false) != hipSuccess) return fail(CM_ERR_HIP);
   CM_HIP(hipMemcpyAsync(b->warn, warn_keep, sizeof(warn_keep), hipMemcpyHostToDevice, b->stream));
@@ -1728,7 +1728,7 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
   if (chosen) {
     chosen[0] = b->tile_v;
     chosen[1] = b->lb.trav_chunk;
-    chosen[2] = b->nb.secant_grid;
+    chosen[2] = b->lb.trav_budget;
   }
   return CM_OK;
 }

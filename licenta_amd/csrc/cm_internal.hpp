@@ -173,7 +173,6 @@ struct NarrowBufs {
   int* d_nrec = nullptr;
   uint32_t* warn = nullptr;
   void* scan_temp = nullptr;
-  int secant_grid = 4096;  // workgroups of the secant pass (launch shape only)
 };
 int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
                   const float4* v, float th, float dt, int ee_on, hipStream_t s);

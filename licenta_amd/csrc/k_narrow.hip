@@ -195,23 +195,6 @@ __device__ __forceinline__ float triple_scalar(f3 x21, f3 x31, f3 x41, f3 v21, f
   return dot(cross(add(x21, mul(v21, t)), add(x31, mul(v31, t))), add(x41, mul(v41, t)));
 }
 
-// Secant search on [a, b] (Physics/CudaCubicSolver.cu:131-153, 20 steps). The
-// reference evaluates f at both points every step; f(x0) of a step is f(x1) of the
-// previous one, computed from the same inputs, so it is carried over unchanged.
-__device__ __forceinline__ float secant(f3 x21, f3 x31, f3 x41, f3 v21, f3 v31, f3 v41, float a, float b) {
-  float x0 = a, x1 = b;
-  float f0 = triple_scalar(x21, x31, x41, v21, v31, v41, x0);
-  for (int i = 0; i < 20; ++i) {
-    const float f1 = triple_scalar(x21, x31, x41, v21, v31, v41, x1);
-    const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
-    if (fabsf(xc - x1) < kCubicEps) return xc;
-    x0 = x1;
-    f0 = f1;
-    x1 = xc;
-  }
-  return -15.f;
-}
-
 // The coplanarity cubic of Physics/CudaCubicSolver.cu:5-153: relative positions and
 // velocities of vertices 2-4 against vertex 1, and the critical points in [0, 1]
 // that split [0, 1] into cc + 1 intervals (ends 0, [cp1], [cp2], 1).
@@ -369,11 +352,9 @@ __device__ __forceinline__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, floa
 // four positions cumulatively by root * dt, and runs the VF / EE proximity test
 // after each root until one hits. The cubic depends on x and v only, so the secant
 // roots are independent of the tests and are found here in a pass of their own:
-//   k_ccd_plan    per candidate: cubic, f at the interval ends, plan word
-//   k_ccd_secant  every wave runs the secant searches of its range of candidates,
-//                 each lane taking the next candidate when its searches end, so a
-//                 long search does not idle the others; root of interval i of
-//                 candidate id in roots[3 * id + i]
+//   k_ccd_plan    per candidate: cubic, f at the interval ends, plan word, and the
+//                 secant root of each interval with a sign change (interval i of
+//                 candidate id in roots[3 * id + i])
 //   k_ccd_near    per candidate: the roots in order, cumulative advance, box-gap
 //                 rejection only; near flag and near count per tile
 //   k_near_list   ids of the near candidates in order
@@ -401,11 +382,33 @@ __device__ __forceinline__ int4 cand_of(int id, int nvf, const int4* __restrict_
   return isvf ? cand_vf[id] : cand_ee[id - nvf];
 }
 
+// Secant search on [a, b] (Physics/CudaCubicSolver.cu:131-153, 20 steps), from the
+// state x0 = a, x1 = b, f0 = f(a). The reference evaluates f at both points every
+// step; f(x0) of a step is f(x1) of the previous one, computed from the same inputs,
+// so it is carried over unchanged. secant_steps runs k steps: true with the root when
+// the search converges within them (the reference returns -15 after 20).
+__device__ __forceinline__ bool secant_steps(const Cubic& q, float& x0, float& x1, float& f0,
+                                             int k, float& root) {
+  for (int i = 0; i < k; ++i) {
+    const float f1 = cubic_f(q, x1);
+    const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
+    if (fabsf(xc - x1) < kCubicEps) {
+      root = xc;
+      return true;
+    }
+    x0 = x1;
+    f0 = f1;
+    x1 = xc;
+  }
+  return false;
+}
+
 __global__ void __launch_bounds__(kBlock) k_ccd_plan(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
     const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
     int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v,
-    int* __restrict__ plan, float2* __restrict__ cps, int* __restrict__ d_ntotal) {
+    int* __restrict__ plan, float2* __restrict__ cps, float* __restrict__ roots,
+    int* __restrict__ d_ntotal) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
   const int total = nvf + nee;
   if (blockIdx.x == 0 && threadIdx.x == 0) *d_ntotal = total;
@@ -428,94 +431,20 @@ __global__ void __launch_bounds__(kBlock) k_ccd_plan(
         fa = fb;
       }
       cp = make_float2(q.cp1, q.cp2);
+      // The secant root of each interval with a sign change, with the cubic already in
+      // registers. A pass of its own (lanes refilled from a per-wave range as their
+      // searches ended) cost more in reloading candidates than the searches do: on
+      // the draped sphere drop 92% of them converge in two steps.
+      for (int i = 0; i <= q.cc; ++i) {
+        if (!(pl & (1 << (5 + i)))) continue;
+        float x0 = i == 0 ? 0.f : cubic_end(q.cp1, q.cp2, q.cc, i - 1);
+        float x1 = cubic_end(q.cp1, q.cp2, q.cc, i);
+        float f0 = cubic_f(q, x0), r;
+        roots[3 * (int64_t)id + i] = secant_steps(q, x0, x1, f0, 20, r) ? r : kNoRoot;
+      }
     }
     plan[id] = pl;
     cps[id] = cp;
-  }
-}
-
-// Secant searches (Physics/CudaCubicSolver.cu:131-153; f(x0) carried over from the
-// previous step, see secant()). Wave w owns candidates [w*per, (w+1)*per); whenever
-// at least kRefill lanes are idle they take the next candidates of the range
-// together, so most steps of the loop are one secant iteration on most lanes.
-constexpr int kRefill = 24;
-
-__global__ void __launch_bounds__(kBlock) k_ccd_secant(
-    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
-    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
-    const int* __restrict__ plan, const float2* __restrict__ cps, const float4* __restrict__ x,
-    const float4* __restrict__ v, float* __restrict__ roots) {
-  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
-  const int lane = threadIdx.x & 63;
-  const int nwaves = gridDim.x * (kBlock / 64);
-  const int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const int per = (total + nwaves - 1) / nwaves;
-  int next = min(w * per, total);
-  const int end = min(next + per, total);
-  const uint64_t below = (1ull << lane) - 1ull;
-  bool active = false;
-  int id = 0, pl = 0, ivl = 0, it = 0;
-  float2 cp = make_float2(0.f, 0.f);
-  float x0 = 0.f, x1 = 0.f, f0 = 0.f;
-  Cubic q;
-  // next interval of the current candidate with a sign change, from ivl on
-  auto start_interval = [&]() {
-    while (ivl <= (pl & 3) && !(pl & (1 << (5 + ivl)))) ++ivl;
-    if (ivl > (pl & 3)) {
-      active = false;
-      return;
-    }
-    x0 = ivl == 0 ? 0.f : cubic_end(cp.x, cp.y, pl & 3, ivl - 1);
-    x1 = cubic_end(cp.x, cp.y, pl & 3, ivl);
-    f0 = cubic_f(q, x0);
-    it = 0;
-  };
-  for (;;) {
-    const uint64_t idle = __ballot(!active);
-    const int nidle = __popcll(idle);
-    if (next < end && (nidle >= kRefill || nidle == 64)) {
-      const int r = __popcll(idle & below);
-      if (!active && next + r < end) {
-        id = next + r;
-        pl = plan[id];
-        if (!(pl & kPlanDead) && (pl & kPlanSecants)) {
-          bool isvf;
-          const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
-          const int4 q4 = cubic_ids(isvf, c);
-          cp = cps[id];
-          const f3 a1 = xyz(x[q4.x]), b1 = xyz(v[q4.x]);
-          q.x21 = sub(xyz(x[q4.y]), a1);
-          q.x31 = sub(xyz(x[q4.z]), a1);
-          q.x41 = sub(xyz(x[q4.w]), a1);
-          q.v21 = sub(xyz(v[q4.y]), b1);
-          q.v31 = sub(xyz(v[q4.z]), b1);
-          q.v41 = sub(xyz(v[q4.w]), b1);
-          ivl = 0;
-          active = true;
-          start_interval();
-        }
-      }
-      next = min(next + nidle, end);
-    }
-    if (!__any(active)) {
-      if (next >= end) break;
-      continue;
-    }
-    if (active) {
-      const float f1 = cubic_f(q, x1);
-      const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
-      ++it;
-      const bool conv = fabsf(xc - x1) < kCubicEps;
-      if (conv || it == 20) {
-        roots[3 * (int64_t)id + ivl] = conv ? xc : kNoRoot;
-        ++ivl;
-        start_interval();
-      } else {
-        x0 = x1;
-        f0 = f1;
-        x1 = xc;
-      }
-    }
   }
 }
 
@@ -911,8 +840,6 @@ __global__ void k_copy_i32(const int* __restrict__ s, int* __restrict__ d, int n
 
 }  // namespace
 
-constexpr int kSecantGrid = 4096;  // persistent workgroups of the secant pass (default)
-
 static int grid_for(int64_t cap) {
   int64_t g = (cap + kBlock - 1) / kBlock;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, 2048));
@@ -942,10 +869,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   const int gc = grid_for(ccap);
   hipLaunchKernelGGL(k_ccd_plan, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
                      nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, nb.cplan, nb.ccp,
-                     nb.d_ntotal);
-  hipLaunchKernelGGL(k_ccd_secant, dim3(nb.secant_grid > 0 ? nb.secant_grid : kSecantGrid),
-                     dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
-                     nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp, x, v, nb.croot);
+                     nb.croot, nb.d_ntotal);
   hipLaunchKernelGGL(k_ccd_near, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp,
                      nb.croot, x, v, th, dt, nb.hit_flag, nb.tile_sum);
