@@ -6,6 +6,7 @@
 
     python -m licenta_amd.workload --adjoint --size 128 --substeps 60 --every 10
     python -m licenta_amd.workload --batch 512 --substeps 8
+    python -m licenta_amd.workload --batch 512 --substeps 290 --kernel-reps 0 --save b.npz
 
 Creates the 256x256 sphere-drop cloth, writes its initial state back with
 cm_set_state (which synchronises the body's stream before the substep graph is
@@ -89,12 +90,31 @@ def main() -> None:
 def batch(args) -> None:
     scenes = configs.batched_scene(2, count=args.batch)
     body = ClothBody.batch(scenes)
-    x0 = np.concatenate([s["xyz"] for s in scenes])
-    body.set_state(x0, np.zeros_like(x0))
+    start = 0
+    if args.load:
+        st = np.load(args.load)
+        body.set_state(st["x"], st["v"], p=st["p"], tri=st["tri"])
+        start = int(st["substep"])
+    else:
+        x0 = np.concatenate([s["xyz"] for s in scenes])
+        body.set_state(x0, np.zeros_like(x0))
     body.fixed_update(args.substeps)
     body.sync()
-    print(json.dumps(dict(workload=f"batch of {args.batch} cloths", substeps=args.substeps,
-                          counts=body.counts(), warnings=body.warnings())))
+    if args.save:
+        np.savez(args.save, x=body.buffer("x")[:, :3], v=body.buffer("v")[:, :3],
+                 p=body.buffer("p")[:, :3], tri=body.buffer("tri"),
+                 substep=start + args.substeps)
+    trav = C.c_float()
+    if args.time_traverse > 0 and lib().cm_time_kernel(body.h, 2, args.time_traverse,
+                                                       C.byref(trav)):
+        raise RuntimeError("cm_time_kernel(traverse) failed")
+    avg = C.c_float()
+    if args.kernel_reps > 0 and lib().cm_time_kernel(body.h, 0, args.kernel_reps, C.byref(avg)):
+        raise RuntimeError("cm_time_kernel failed")
+    print(json.dumps(dict(workload=f"batch of {args.batch} cloths", start_substep=start,
+                          substeps=args.substeps, solve_avg_ms=float(avg.value),
+                          traverse_avg_ms=float(trav.value), counts=body.counts(),
+                          warnings=body.warnings())))
     body.close()
 
 
