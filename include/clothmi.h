@@ -96,6 +96,14 @@ extern "C" {
  * (Physics/CudaDeformableBody.cpp:583), so it is off by default. Not with
  * cm_forward_ckpt. */
 #define CM_F_REVERT_TOC (1u << 10)
+/* Discrete instead of continuous narrow phase: each VF / EE candidate is tested
+ * once at the substep's start positions x, with the same proximity tests and no
+ * coplanarity roots (CudaPrimitiveTests::DCDTriangleTests, Collision/
+ * CudaPrimitiveTests.cu:7-94, reached from HandleCollisionsDiscrete, whose call the
+ * reference's FixedUpdate comments out: Physics/CudaDeformableBody.cpp:419-439,
+ * 611). A contact's time is what the test leaves: 0, or -1 where the reference's
+ * TestEE does not set it. */
+#define CM_F_DCD (1u << 11)
 
 /* ---- reference-compatibility bits (cm_params.ref_compat) ----------------- */
 /* Leaf AABBs stay in pre-sort triangle slots after the Morton sort

@@ -35,6 +35,7 @@ F_FRICTION = 1 << 7
 F_BEND_FP64 = 1 << 8
 F_VERTEX_NORMALS = 1 << 9
 F_REVERT_TOC = 1 << 10
+F_DCD = 1 << 11
 
 RC_STALE_LEAF_AABB = 1 << 0
 RC_BEND_ABS = 1 << 1

@@ -508,7 +508,7 @@ int record_substep(cm_body* b, hipEvent_t* ev = nullptr, const TapeSlot* tape = 
   mark(CM_STAGE_NARROW);
   if (coll) {
     rc = launch_narrow(b->nb, b->tri_alt, b->x, b->p, b->v, P.thickness, P.timestep,
-                       (P.flags & CM_F_EE_CONTACTS) ? 1 : 0, s);
+                       (P.flags & CM_F_EE_CONTACTS) ? 1 : 0, (P.flags & CM_F_DCD) ? 1 : 0, s);
     if (rc) return rc;
     rc = launch_dynamic_csr(b->nb, s);
     if (rc) return rc;

@@ -174,8 +174,9 @@ struct NarrowBufs {
   uint32_t* warn = nullptr;
   void* scan_temp = nullptr;
 };
+// dcd: one proximity test per candidate at x instead of the CCD (CM_F_DCD)
 int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
-                  const float4* v, float th, float dt, int ee_on, hipStream_t s);
+                  const float4* v, float th, float dt, int ee_on, int dcd, hipStream_t s);
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s);
 
 // ---- PBD (k_pbd.hip) -------------------------------------------------------

@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_plan(
     const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
     int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v,
     int* __restrict__ plan, float2* __restrict__ cps, float* __restrict__ roots,
-    int* __restrict__ d_ntotal) {
+    int* __restrict__ d_ntotal, int dcd) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
   const int total = nvf + nee;
   if (blockIdx.x == 0 && threadIdx.x == 0) *d_ntotal = total;
@@ -417,7 +417,9 @@ __global__ void __launch_bounds__(kBlock) k_ccd_plan(
     const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
     int pl = kPlanDead;
     float2 cp = make_float2(0.f, 0.f);
-    if (sane(isvf, c)) {
+    if (dcd) {  // one test at x, no cubic
+      if (sane(isvf, c)) pl = 0;
+    } else if (sane(isvf, c)) {
       const int4 q4 = cubic_ids(isvf, c);
       Cubic q;
       cubic_setup(xyz(x[q4.x]), xyz(x[q4.y]), xyz(x[q4.z]), xyz(x[q4.w]), xyz(v[q4.x]),
@@ -475,13 +477,16 @@ __device__ __forceinline__ bool ccd_tests(bool isvf, int4 c, int pl, float2 cp, 
                                           const float* __restrict__ roots,
                                           const float4* __restrict__ x,
                                           const float4* __restrict__ v, float th, float dt,
-                                          Hit& h) {
-  h.t = -1.f;
+                                          int dcd, Hit& h) {
+  h.t = -1.f;  // Physics/Structs.h:122-128
   h.w[0] = h.w[1] = h.w[2] = h.w[3] = -1.f;
   h.n = mk3(-1.f, -1.f, -1.f);
   if (pl & kPlanDead) return false;
   const int4 q4 = cubic_ids(isvf, c);
   const f3 o1 = xyz(x[q4.x]), o2 = xyz(x[q4.y]), o3 = xyz(x[q4.z]), o4 = xyz(x[q4.w]);
+  if (dcd)  // Collision/CudaPrimitiveTests.cu:41-94: the tests at x; t as they leave it
+    return isvf ? !vf_far(o4, o1, o2, o3, th) && test_vf(o4, o1, o2, o3, h, th)
+                : !ee_far(o1, o2, o3, o4, th) && test_ee(o1, o2, o3, o4, h, th);
   const f3 u1 = xyz(v[q4.x]), u2 = xyz(v[q4.y]), u3 = xyz(v[q4.z]), u4 = xyz(v[q4.w]);
   f3 p1 = o1, p2 = o2, p3 = o3, p4 = o4;  // advanced cumulatively by each root
   // one root: advance (Collision/CudaPrimitiveTests.cu:164-171), then test
@@ -523,10 +528,12 @@ __device__ __forceinline__ bool ccd_tests(bool isvf, int4 c, int pl, float2 cp, 
 __device__ __forceinline__ bool ccd_near(bool isvf, int4 c, int pl, float2 cp, int id,
                                          const float* __restrict__ roots,
                                          const float4* __restrict__ x,
-                                         const float4* __restrict__ v, float th, float dt) {
+                                         const float4* __restrict__ v, float th, float dt,
+                                         int dcd) {
   if (pl & kPlanDead) return false;
   const int4 q4 = cubic_ids(isvf, c);
   f3 p1 = xyz(x[q4.x]), p2 = xyz(x[q4.y]), p3 = xyz(x[q4.z]), p4 = xyz(x[q4.w]);
+  if (dcd) return isvf ? !vf_far(p4, p1, p2, p3, th) : !ee_far(p1, p2, p3, p4, th);
   const f3 u1 = xyz(v[q4.x]), u2 = xyz(v[q4.y]), u3 = xyz(v[q4.z]), u4 = xyz(v[q4.w]);
   auto near_at = [&](float root) {
     const float t = root * dt;
@@ -562,7 +569,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_near(
     const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
     const int* __restrict__ plan, const float2* __restrict__ cps,
     const float* __restrict__ roots, const float4* __restrict__ x, const float4* __restrict__ v,
-    float th, float dt, uint8_t* __restrict__ near_flag, int* __restrict__ tile_sum) {
+    float th, float dt, int dcd, uint8_t* __restrict__ near_flag, int* __restrict__ tile_sum) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
   const int ntiles = (total + (64 * kBig) - 1) / (64 * kBig);
   const int lane = threadIdx.x & 63;
@@ -575,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_near(
       if (id < total) {
         bool isvf;
         const int4 q = cand_of(id, nvf, cand_vf, cand_ee, isvf);
-        const bool nr = ccd_near(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt);
+        const bool nr = ccd_near(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt, dcd);
         near_flag[id] = (uint8_t)nr;
         c += nr;
       }
@@ -616,7 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_test(
     const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_nnear,
     const int* __restrict__ near_ids, const int* __restrict__ plan,
     const float2* __restrict__ cps, const float* __restrict__ roots,
-    const float4* __restrict__ x, const float4* __restrict__ v, float th, float dt,
+    const float4* __restrict__ x, const float4* __restrict__ v, float th, float dt, int dcd,
     uint8_t* __restrict__ hit_flag, int* __restrict__ tile_sum) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nn = *d_nnear;
   const int ntiles = (nn + (64 * kSmall) - 1) / (64 * kSmall);
@@ -632,7 +639,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_test(
         bool isvf;
         const int4 q = cand_of(id, nvf, cand_vf, cand_ee, isvf);
         Hit h;
-        const bool hit = ccd_tests(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt, h);
+        const bool hit = ccd_tests(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt, dcd, h);
         hit_flag[k] = (uint8_t)hit;
         if (hit) c += isvf ? 1 : (1 << 16);
       }
@@ -651,9 +658,9 @@ __global__ void __launch_bounds__(kBlock) k_ccd_write(
     const float2* __restrict__ cps, const float* __restrict__ roots,
     const uint8_t* __restrict__ hit_flag, const int* __restrict__ tile_pre, int64_t vf_out_cap,
     int64_t ee_out_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
-    float dt, int4* __restrict__ vf_v, float4* __restrict__ vf_w, float4* __restrict__ vf_tn,
-    int4* __restrict__ ee_v, float4* __restrict__ ee_w, float4* __restrict__ ee_tn,
-    uint32_t* __restrict__ warn) {
+    float dt, int dcd, int4* __restrict__ vf_v, float4* __restrict__ vf_w,
+    float4* __restrict__ vf_tn, int4* __restrict__ ee_v, float4* __restrict__ ee_w,
+    float4* __restrict__ ee_tn, uint32_t* __restrict__ warn) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nn = *d_nnear;
   const int ntiles = (nn + (64 * kSmall) - 1) / (64 * kSmall);
   const int lane = threadIdx.x & 63;
@@ -681,7 +688,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_write(
         continue;
       }
       Hit h;
-      ccd_tests(isvf, q, plan[id[j]], cps[id[j]], id[j], roots, x, v, th, dt, h);
+      ccd_tests(isvf, q, plan[id[j]], cps[id[j]], id[j], roots, x, v, th, dt, dcd, h);
       const float4 w4 = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
       const float4 tn = make_float4(h.t, h.n.x, h.n.y, h.n.z);
       if (isvf) { vf_v[o] = q; vf_w[o] = w4; vf_tn[o] = tn; }
@@ -852,7 +859,7 @@ static int tile_grid(int64_t items, int R) {
 }
 
 int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
-                  const float4* v, float th, float dt, int ee_on, hipStream_t s) {
+                  const float4* v, float th, float dt, int ee_on, int dcd, hipStream_t s) {
   // candidates: masks and tile counts, tile prefixes (and the clamped totals), writes
   const int ptiles = (int)((nb.pair_cap + 64 * kSmall - 1) / (64 * kSmall));
   hipLaunchKernelGGL(k_cand_count, dim3(tile_grid(nb.pair_cap, kSmall)), dim3(kBlock), 0, s, nb.pairs,
@@ -869,10 +876,10 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   const int gc = grid_for(ccap);
   hipLaunchKernelGGL(k_ccd_plan, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
                      nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, nb.cplan, nb.ccp,
-                     nb.croot, nb.d_ntotal);
+                     nb.croot, nb.d_ntotal, dcd);
   hipLaunchKernelGGL(k_ccd_near, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp,
-                     nb.croot, x, v, th, dt, nb.hit_flag, nb.tile_sum);
+                     nb.croot, x, v, th, dt, dcd, nb.hit_flag, nb.tile_sum);
   CM_LAUNCH_CHECK();
   const int ctiles = (int)((ccap + 64 * kBig - 1) / (64 * kBig));
   rc = scan_tiles(nb.d_ntotal, 64 * kBig, ctiles, nb.tile_sum, nb.tile_pre, 0, ccap, 0,
@@ -882,7 +889,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
                      nb.hit_flag, nb.tile_pre, nb.near_ids);
   hipLaunchKernelGGL(k_ccd_test, dim3(tile_grid(ccap, kSmall)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,
-                     nb.ccp, nb.croot, x, v, th, dt, nb.hit_flag, nb.tile_sum);
+                     nb.ccp, nb.croot, x, v, th, dt, dcd, nb.hit_flag, nb.tile_sum);
   CM_LAUNCH_CHECK();
   const int ntiles_near = (int)((ccap + 64 * kSmall - 1) / (64 * kSmall));
   rc = scan_tiles(nb.d_nnear, 64 * kSmall, ntiles_near, nb.tile_sum, nb.tile_pre, 1, nb.vf_out_cap,
@@ -891,7 +898,8 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   hipLaunchKernelGGL(k_ccd_write, dim3(tile_grid(ccap, kSmall)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,
                      nb.ccp, nb.croot, nb.hit_flag, nb.tile_pre, nb.vf_out_cap, nb.ee_out_cap, x,
-                     v, th, dt, nb.vf_v, nb.vf_w, nb.vf_tn, nb.ee_v, nb.ee_w, nb.ee_tn, nb.warn);
+                     v, th, dt, dcd, nb.vf_v, nb.vf_w, nb.vf_tn, nb.ee_v, nb.ee_w, nb.ee_tn,
+                     nb.warn);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }

@@ -601,6 +601,20 @@ inline bool ccd_ee(cm_contact& c, const F3* x, const F3* v, float th, float dt) 
   return false;
 }
 
+// Collision/CudaPrimitiveTests.cu:41-94 (DCDTestVFs, DCDTestEEsBridson): the same
+// sanity checks and proximity tests once, at the positions x; the contact's time is
+// what the test leaves (0, or the candidate's -1 where TestEE does not set it).
+inline bool dcd_vf(cm_contact& c, const F3* x, float th) {
+  int p = c.v[0], a = c.v[1], b = c.v[2], d = c.v[3];
+  if (p == a || p == b || p == d) return false;  // VFSanity :240-243
+  return test_vf(x[p], x[a], x[b], x[d], c, th);
+}
+inline bool dcd_ee(cm_contact& c, const F3* x, float th) {
+  int p1 = c.v[0], p2 = c.v[1], q1 = c.v[2], q2 = c.v[3];
+  if (p1 == q1 || p1 == q2 || p2 == q1 || p2 == q2) return false;  // EESanity :245-248
+  return test_ee(x[p1], x[p2], x[q1], x[q2], c, th, 0.000000001f);
+}
+
 inline cm_contact make_contact(int a, int b, int c, int d) {  // Physics/Structs.h:112-128
   cm_contact r;
   r.v[0] = a;
@@ -1119,11 +1133,14 @@ struct Sim {
       eec.insert(eec.end(), pee[id].begin(), pee[id].end());
     }
     const float dt = prm.timestep;
+    const bool dcd = (prm.flags & CM_F_DCD) != 0;  // HandleCollisionsDiscrete's tests
     std::vector<uint8_t> hvf(vfc.size()), hee(eec.size());
 #pragma omp parallel for schedule(dynamic, 512)
-    for (long k = 0; k < (long)vfc.size(); ++k) hvf[k] = ccd_vf(vfc[k], xf.data(), vel.data(), th, dt);
+    for (long k = 0; k < (long)vfc.size(); ++k)
+      hvf[k] = dcd ? dcd_vf(vfc[k], xf.data(), th) : ccd_vf(vfc[k], xf.data(), vel.data(), th, dt);
 #pragma omp parallel for schedule(dynamic, 512)
-    for (long k = 0; k < (long)eec.size(); ++k) hee[k] = ccd_ee(eec[k], xf.data(), vel.data(), th, dt);
+    for (long k = 0; k < (long)eec.size(); ++k)
+      hee[k] = dcd ? dcd_ee(eec[k], xf.data(), th) : ccd_ee(eec[k], xf.data(), vel.data(), th, dt);
     for (size_t k = 0; k < vfc.size(); ++k)
       if (hvf[k]) vf.push_back(vfc[k]);
     for (size_t k = 0; k < eec.size(); ++k)

@@ -79,6 +79,8 @@ def compare_substep(oracle_mod, scene, body, check_tree=True, tol64=TOL64, dev=F
         ovf, oee = o32.contacts(False), o32.contacts(True)
         assert np.array_equal(gvf["v"], ovf["v"]) and np.array_equal(gee["v"], oee["v"])
         assert np.array_equal(gvf["w"], ovf["w"]) and np.array_equal(gee["w"], oee["w"])
+        assert np.array_equal(gvf["t"], ovf["t"]) and np.array_equal(gee["t"], oee["t"])
+        assert np.array_equal(gvf["n"], ovf["n"]) and np.array_equal(gee["n"], oee["n"])
         res.update(pairs=len(gp), vf=len(gvf), ee=len(gee))
         # the FP64 oracle's discrete stages run on the FP32 shadow: same pairs
         assert np.array_equal(o64.pairs(), gp)
@@ -192,6 +194,41 @@ def test_folded_cloth_contacts(oracle_mod):
         total += r["vf"] + r["ee"]
     body.close()
     assert total > 0
+
+
+def test_folded_cloth_dcd(oracle_mod):
+    """The discrete narrow phase (CM_F_DCD): the folded cloth's candidates tested
+    once at the start positions, three substeps against the oracle (contact sets bit
+    for bit, positions), with other contacts than the continuous phase finds on the
+    same first substep (the flag switches the path: 103 discrete against 133
+    continuous contacts on the first substep)."""
+    n = 24
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    h = n // 2
+    for i in range(h, n):
+        src = 2 * h - 1 - i
+        xyz[i, :, 0] = xyz[src, :, 0] + 0.002
+        xyz[i, :, 1] = 0.003 + 0.0005 * (i - h)
+    sc["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    sc["pinned"] = None
+    sc["inv_mass"] = np.full(n * n, 1000.0, np.float32)
+    from licenta_amd.body import ClothBody
+    v = np.zeros((n * n, 3), np.float32)
+    v.reshape(n, n, 3)[h:, :, 1] = -0.3
+    first = {}
+    for name, flags in (("ccd", sc["params"]["flags"]),
+                        ("dcd", sc["params"]["flags"] | configs.F_DCD)):
+        s2 = dict(sc, params=dict(sc["params"], thickness=0.004, flags=flags))
+        body = ClothBody.from_scene(s2)
+        body.set_state(s2["xyz"], v)
+        for k in range(3):
+            r = compare_substep(oracle_mod, s2, body)
+            if k == 0:
+                first[name] = r["vf"] + r["ee"]
+        body.close()
+    print(f"  first-substep contacts: continuous {first['ccd']}, discrete {first['dcd']}")
+    assert first["dcd"] > 0 and first["ccd"] > 0 and first["dcd"] != first["ccd"]
 
 
 @pytest.mark.parametrize("tail", [True, False])
