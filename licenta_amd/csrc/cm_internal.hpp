@@ -155,8 +155,9 @@ struct NarrowBufs {
   uint8_t* hit_flag = nullptr;  // per candidate (VF then EE): near, then hit per near index
   int* near_ids = nullptr;       // candidates passing the box-gap rejection, in order
   int* d_nnear = nullptr;
-  // CCD passes, per candidate (VF then EE, vf_cap + ee_cap): plan word, critical
-  // points, and the secant root of each of its (up to three) intervals
+  // CCD passes, per candidate (VF then EE, vf_cap + ee_cap), written for near
+  // candidates only: plan word, critical points, and the secant root of each of its
+  // (up to three) intervals
   int* cplan = nullptr;
   float2* ccp = nullptr;
   float* croot = nullptr;  // 3 * (vf_cap + ee_cap)

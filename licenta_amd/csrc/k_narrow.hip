@@ -350,20 +350,20 @@ __device__ __forceinline__ bool test_ee(f3 x1, f3 x2, f3 x3, f3 x4, Hit& h, floa
 // finds the coplanarity roots in [0, 1] in interval order (a root at an interval
 // start when |f| < EPS there, then a secant root when f changes sign), advances all
 // four positions cumulatively by root * dt, and runs the VF / EE proximity test
-// after each root until one hits. The cubic depends on x and v only, so the secant
-// roots are independent of the tests and are found here in a pass of their own:
-//   k_ccd_plan    per candidate: cubic, f at the interval ends, plan word, and the
-//                 secant root of each interval with a sign change (interval i of
-//                 candidate id in roots[3 * id + i])
-//   k_ccd_near    per candidate: the roots in order, cumulative advance, box-gap
-//                 rejection only; near flag and near count per tile
+// after each root until one hits. Here:
+//   k_ccd_plan    per candidate: cubic, f at the interval ends, plan word, the secant
+//                 root of each interval with a sign change, then the roots in order
+//                 with the same cumulative advance and only the box-gap rejection:
+//                 a near flag per candidate, near counts per tile. Plan word,
+//                 critical points and roots (interval i of candidate id in
+//                 roots[3 * id + i]) are stored for near candidates only, the only
+//                 ones the later passes read.
 //   k_near_list   ids of the near candidates in order
 //   k_ccd_test    per near candidate: the full tests; hit flag and hit counts
 //   k_ccd_write   hits at their ordered positions in the VF / EE contact lists
 // Plan word: bits 0-1 cc, bits 2-4 root at the start of interval i, bits 5-7 sign
 // change on interval i, bit 8 nothing to test (degenerate candidate).
 constexpr int kPlanDead = 1 << 8;
-constexpr int kPlanSecants = 7 << 5;
 constexpr float kNoRoot = -15.f;  // Physics/CudaCubicSolver.cu:152
 
 // Vertex ids of candidate c in the cubic's order (x1, x2, x3, x4): VF candidates are
@@ -382,71 +382,48 @@ __device__ __forceinline__ int4 cand_of(int id, int nvf, const int4* __restrict_
   return isvf ? cand_vf[id] : cand_ee[id - nvf];
 }
 
-// Secant search on [a, b] (Physics/CudaCubicSolver.cu:131-153, 20 steps), from the
-// state x0 = a, x1 = b, f0 = f(a). The reference evaluates f at both points every
-// step; f(x0) of a step is f(x1) of the previous one, computed from the same inputs,
-// so it is carried over unchanged. secant_steps runs k steps: true with the root when
-// the search converges within them (the reference returns -15 after 20).
-__device__ __forceinline__ bool secant_steps(const Cubic& q, float& x0, float& x1, float& f0,
-                                             int k, float& root) {
-  for (int i = 0; i < k; ++i) {
-    const float f1 = cubic_f(q, x1);
+// Secant searches (Physics/CudaCubicSolver.cu:131-153, 20 steps) on the intervals
+// with a sign change, from x0 = start, x1 = end, f0 = f(x0), f1 = f(x1). The
+// reference evaluates f at both points every step; f(x0) of a step is f(x1) of the
+// previous one, and the first step's two values are the ones the plan computed at
+// the interval's ends, all from the same inputs, so they are carried over unchanged.
+// A search converges when |xc - x1| < EPS (root xc) or gives up after 20 steps (the
+// reference returns -15). A lane runs its searches one after the other in one loop,
+// so the wave waits for the lane with the most steps in all, not for the slowest
+// lane of every interval in turn.
+__device__ __forceinline__ void secants(const Cubic& q, int pl, const float (&fe)[4],
+                                        float (&r)[3]) {
+  auto next = [&](int i) {  // first interval >= i with a sign change, or 3
+    while (i <= q.cc && !(pl & (1 << (5 + i)))) ++i;
+    return i <= q.cc ? i : 3;
+  };
+  auto start = [&](int i, float& x0, float& x1, float& f0, float& f1) {
+    x0 = i == 0 ? 0.f : (i == 1 ? q.cp1 : q.cp2);
+    x1 = cubic_end(q.cp1, q.cp2, q.cc, i);
+    f0 = i == 0 ? fe[0] : (i == 1 ? fe[1] : fe[2]);
+    f1 = i == 0 ? fe[1] : (i == 1 ? fe[2] : fe[3]);
+  };
+  int i = next(0), k = 0;
+  float x0 = 0.f, x1 = 0.f, f0 = 0.f, f1 = 0.f;
+  if (i < 3) start(i, x0, x1, f0, f1);
+  while (i < 3) {
     const float xc = (x0 * f1 - x1 * f0) / (f1 - f0);
-    if (fabsf(xc - x1) < kCubicEps) {
-      root = xc;
-      return true;
+    const bool conv = fabsf(xc - x1) < kCubicEps;
+    if (conv) {
+      r[0] = i == 0 ? xc : r[0];
+      r[1] = i == 1 ? xc : r[1];
+      r[2] = i == 2 ? xc : r[2];
+    }
+    if (conv || ++k == 20) {
+      i = next(i + 1);
+      k = 0;
+      if (i < 3) start(i, x0, x1, f0, f1);
+      continue;
     }
     x0 = x1;
     f0 = f1;
     x1 = xc;
-  }
-  return false;
-}
-
-__global__ void __launch_bounds__(kBlock) k_ccd_plan(
-    const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
-    const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
-    int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v,
-    int* __restrict__ plan, float2* __restrict__ cps, float* __restrict__ roots,
-    int* __restrict__ d_ntotal, int dcd) {
-  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
-  const int total = nvf + nee;
-  if (blockIdx.x == 0 && threadIdx.x == 0) *d_ntotal = total;
-  for (int id = blockIdx.x * kBlock + threadIdx.x; id < total; id += gridDim.x * kBlock) {
-    bool isvf;
-    const int4 c = cand_of(id, nvf, cand_vf, cand_ee, isvf);
-    int pl = kPlanDead;
-    float2 cp = make_float2(0.f, 0.f);
-    if (dcd) {  // one test at x, no cubic
-      if (sane(isvf, c)) pl = 0;
-    } else if (sane(isvf, c)) {
-      const int4 q4 = cubic_ids(isvf, c);
-      Cubic q;
-      cubic_setup(xyz(x[q4.x]), xyz(x[q4.y]), xyz(x[q4.z]), xyz(x[q4.w]), xyz(v[q4.x]),
-                  xyz(v[q4.y]), xyz(v[q4.z]), xyz(v[q4.w]), q);
-      pl = q.cc;
-      float fa = cubic_f(q, 0.f);
-      for (int i = 0; i <= q.cc; ++i) {
-        const float fb = cubic_f(q, cubic_end(q.cp1, q.cp2, q.cc, i));
-        if (fabsf(fa) < kCubicEps) pl |= 1 << (2 + i);
-        if (fa * fb < 0.f) pl |= 1 << (5 + i);
-        fa = fb;
-      }
-      cp = make_float2(q.cp1, q.cp2);
-      // The secant root of each interval with a sign change, with the cubic already in
-      // registers. A pass of its own (lanes refilled from a per-wave range as their
-      // searches ended) cost more in reloading candidates than the searches do: on
-      // the draped sphere drop 92% of them converge in two steps.
-      for (int i = 0; i <= q.cc; ++i) {
-        if (!(pl & (1 << (5 + i)))) continue;
-        float x0 = i == 0 ? 0.f : cubic_end(q.cp1, q.cp2, q.cc, i - 1);
-        float x1 = cubic_end(q.cp1, q.cp2, q.cc, i);
-        float f0 = cubic_f(q, x0), r;
-        roots[3 * (int64_t)id + i] = secant_steps(q, x0, x1, f0, 20, r) ? r : kNoRoot;
-      }
-    }
-    plan[id] = pl;
-    cps[id] = cp;
+    f1 = cubic_f(q, x1);
   }
 }
 
@@ -521,20 +498,42 @@ __device__ __forceinline__ bool ccd_tests(bool isvf, int4 c, int pl, float2 cp, 
   return false;
 }
 
-// The candidate's roots in order with the same cumulative advance as ccd_tests, but
-// only the box-gap rejection at each root: false when every root is rejected, so the
-// full tests cannot hit. Most candidates end here, and the full tests run on the few
-// others without the divergence of the whole set.
-__device__ __forceinline__ bool ccd_near(bool isvf, int4 c, int pl, float2 cp, int id,
-                                         const float* __restrict__ roots,
-                                         const float4* __restrict__ x,
-                                         const float4* __restrict__ v, float th, float dt,
-                                         int dcd) {
-  if (pl & kPlanDead) return false;
+// One candidate of k_ccd_plan: the plan of its roots and whether any root survives
+// the box-gap rejection after the same cumulative advance as ccd_tests (false: the
+// full tests cannot hit). Stores plan word, critical points and roots when near.
+__device__ __forceinline__ bool plan_near(bool isvf, int4 c, int id, const float4* __restrict__ x,
+                                          const float4* __restrict__ v, float th, float dt,
+                                          int dcd, int* __restrict__ plan,
+                                          float2* __restrict__ cps, float* __restrict__ roots) {
+  if (!sane(isvf, c)) return false;
   const int4 q4 = cubic_ids(isvf, c);
-  f3 p1 = xyz(x[q4.x]), p2 = xyz(x[q4.y]), p3 = xyz(x[q4.z]), p4 = xyz(x[q4.w]);
-  if (dcd) return isvf ? !vf_far(p4, p1, p2, p3, th) : !ee_far(p1, p2, p3, p4, th);
+  const f3 o1 = xyz(x[q4.x]), o2 = xyz(x[q4.y]), o3 = xyz(x[q4.z]), o4 = xyz(x[q4.w]);
+  if (dcd) {  // one test at x, no cubic
+    const bool nr = isvf ? !vf_far(o4, o1, o2, o3, th) : !ee_far(o1, o2, o3, o4, th);
+    if (nr) {
+      plan[id] = 0;
+      cps[id] = make_float2(0.f, 0.f);
+    }
+    return nr;
+  }
   const f3 u1 = xyz(v[q4.x]), u2 = xyz(v[q4.y]), u3 = xyz(v[q4.z]), u4 = xyz(v[q4.w]);
+  Cubic q;
+  cubic_setup(o1, o2, o3, o4, u1, u2, u3, u4, q);
+  // f at the interval ends: fe[i] at the start of interval i, fe[i + 1] at its end
+  int pl = q.cc;
+  float fe[4];
+  fe[0] = cubic_f(q, 0.f);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (i > q.cc) break;
+    fe[i + 1] = cubic_f(q, cubic_end(q.cp1, q.cp2, q.cc, i));
+    if (fabsf(fe[i]) < kCubicEps) pl |= 1 << (2 + i);
+    if (fe[i] * fe[i + 1] < 0.f) pl |= 1 << (5 + i);
+  }
+  // the secant root of each interval with a sign change, with the cubic in registers
+  float r[3] = {kNoRoot, kNoRoot, kNoRoot};
+  secants(q, pl, fe, r);
+  f3 p1 = o1, p2 = o2, p3 = o3, p4 = o4;
   auto near_at = [&](float root) {
     const float t = root * dt;
     if (isvf) {
@@ -550,49 +549,56 @@ __device__ __forceinline__ bool ccd_near(bool isvf, int4 c, int pl, float2 cp, i
     p4 = add(p4, mul(u4, t));
     return !ee_far(p1, p2, p3, p4, th);
   };
-  const int cc = pl & 3;
+  bool nr = false;
   float ea = 0.f;
-  for (int i = 0; i <= cc; ++i) {
-    if ((pl & (1 << (2 + i))) && near_at(ea)) return true;
-    if (pl & (1 << (5 + i))) {
-      const float r = roots[3 * (int64_t)id + i];
-      if (r != kNoRoot && near_at(r)) return true;
-    }
-    ea = cubic_end(cp.x, cp.y, cc, i);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (i > q.cc) break;
+    if ((pl & (1 << (2 + i))) && near_at(ea)) { nr = true; break; }
+    if ((pl & (1 << (5 + i))) && r[i] != kNoRoot && near_at(r[i])) { nr = true; break; }
+    ea = cubic_end(q.cp1, q.cp2, q.cc, i);
   }
-  return false;
+  if (nr) {
+    plan[id] = pl;
+    cps[id] = make_float2(q.cp1, q.cp2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (i <= q.cc && (pl & (1 << (5 + i)))) roots[3 * (int64_t)id + i] = r[i];
+  }
+  return nr;
 }
 
-// pass 4: near flag per candidate (VF then EE ids), near counts per tile of ids
-__global__ void __launch_bounds__(kBlock) k_ccd_near(
+// pass 3: near flag per candidate (VF then EE ids), near counts per tile of ids
+__global__ void __launch_bounds__(kBlock) k_ccd_plan(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
-    const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_ntotal,
-    const int* __restrict__ plan, const float2* __restrict__ cps,
-    const float* __restrict__ roots, const float4* __restrict__ x, const float4* __restrict__ v,
-    float th, float dt, int dcd, uint8_t* __restrict__ near_flag, int* __restrict__ tile_sum) {
-  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), total = *d_ntotal;
+    const int* __restrict__ d_nvf, const int* __restrict__ d_nee, int64_t vf_cap,
+    int64_t ee_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
+    float dt, int dcd, int* __restrict__ plan, float2* __restrict__ cps,
+    float* __restrict__ roots, int* __restrict__ d_ntotal, uint8_t* __restrict__ near_flag,
+    int* __restrict__ tile_sum) {
+  const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nee = (int)min((int64_t)*d_nee, ee_cap);
+  const int total = nvf + nee;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d_ntotal = total;
   const int ntiles = (total + (64 * kBig) - 1) / (64 * kBig);
   const int lane = threadIdx.x & 63;
   for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
        tile += gridDim.x * kWavesPerBlock) {
     int c = 0;
-#pragma unroll
     for (int j = 0; j < kBig; ++j) {
       const int id = tile * (64 * kBig) + j * 64 + lane;
-      if (id < total) {
-        bool isvf;
-        const int4 q = cand_of(id, nvf, cand_vf, cand_ee, isvf);
-        const bool nr = ccd_near(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt, dcd);
-        near_flag[id] = (uint8_t)nr;
-        c += nr;
-      }
+      if (id >= total) break;
+      bool isvf;
+      const int4 q = cand_of(id, nvf, cand_vf, cand_ee, isvf);
+      const bool nr = plan_near(isvf, q, id, x, v, th, dt, dcd, plan, cps, roots);
+      near_flag[id] = (uint8_t)nr;
+      c += nr;
     }
     c = wave_sum(c);
     if (lane == 0) tile_sum[tile] = c;
   }
 }
 
-// pass 5: the near candidates' ids in order
+// pass 4: the near candidates' ids in order
 __global__ void __launch_bounds__(kBlock) k_near_list(const int* __restrict__ d_ntotal,
                                                       const uint8_t* __restrict__ near_flag,
                                                       const int* __restrict__ tile_pre,
@@ -616,7 +622,7 @@ __global__ void __launch_bounds__(kBlock) k_near_list(const int* __restrict__ d_
   }
 }
 
-// pass 6: full tests of the near candidates; hit flag per near index, hit counts
+// pass 5: full tests of the near candidates; hit flag per near index, hit counts
 // (VF low, EE high) per tile of near indices
 __global__ void __launch_bounds__(kBlock) k_ccd_test(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
@@ -649,7 +655,7 @@ __global__ void __launch_bounds__(kBlock) k_ccd_test(
   }
 }
 
-// pass 7 (after the scan of the tile counts): each hit is recomputed and written at
+// pass 6 (after the scan of the tile counts): each hit is recomputed and written at
 // its ordered position among the VF or EE contacts.
 __global__ void __launch_bounds__(kBlock) k_ccd_write(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
@@ -871,15 +877,12 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   hipLaunchKernelGGL(k_cand_write, dim3(tile_grid(nb.pair_cap, kSmall)), dim3(kBlock), 0, s, nb.pairs,
                      nb.d_npairs, nb.pair_cap, tri, nb.mask, nb.tile_pre, nb.cand_vf, nb.cand_ee,
                      nb.vf_cap, nb.ee_cap, nb.warn);
-  // CCD: plan, secant roots, tests with tile hit counts, tile prefixes, writes
+  // CCD: plan with secant roots and near flags, near list, tests with tile hit
+  // counts, tile prefixes, writes
   const int64_t ccap = nb.vf_cap + nb.ee_cap;
-  const int gc = grid_for(ccap);
-  hipLaunchKernelGGL(k_ccd_plan, dim3(gc), dim3(kBlock), 0, s, nb.cand_vf, nb.cand_ee,
-                     nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, nb.cplan, nb.ccp,
-                     nb.croot, nb.d_ntotal, dcd);
-  hipLaunchKernelGGL(k_ccd_near, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
-                     nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_ntotal, nb.cplan, nb.ccp,
-                     nb.croot, x, v, th, dt, dcd, nb.hit_flag, nb.tile_sum);
+  hipLaunchKernelGGL(k_ccd_plan, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
+                     nb.cand_ee, nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, th, dt,
+                     dcd, nb.cplan, nb.ccp, nb.croot, nb.d_ntotal, nb.hit_flag, nb.tile_sum);
   CM_LAUNCH_CHECK();
   const int ctiles = (int)((ccap + 64 * kBig - 1) / (64 * kBig));
   rc = scan_tiles(nb.d_ntotal, 64 * kBig, ctiles, nb.tile_sum, nb.tile_pre, 0, ccap, 0,
