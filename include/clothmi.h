@@ -358,6 +358,62 @@ int cm_lbvh_pairs(int32_t t, const int32_t* d_lefts, const int32_t* d_rights,
                   const int32_t* d_rmlr, int32_t cap, int32_t* d_pairs, int32_t* d_n_pairs,
                   uint32_t* d_warn, void* d_temp, size_t temp_bytes, void* stream);
 
+/* ---- unit-level dead paths of the reference's GPU cloth ------------------ */
+/* Neither is reached by any reference configuration: FixedUpdate's calls of
+ * ClothInternalDynamics and HandleCollisionsContinuous are commented out
+ * (Physics/CudaDeformableBody.cpp:605-613). */
+
+/* Spring network of a rows x cols grid of vertices (vertex i * cols + j), as the
+ * commented-out Physics/CudaDeformableBody.cpp:107-152 builds it with AddSpring
+ * (:288-311), its square dims.x x dims.x loops taken to rows x cols: for each
+ * vertex (i, j), i-major, stretch springs to (i+1, j) and (i, j+1), shear springs
+ * (i, j)-(i+1, j+1) and (i+1, j)-(i, j+1), bend springs to (i+2, j) and (i, j+2);
+ * each spring twice, a -> b listed at a and b -> a listed at b, with the rest
+ * distance of xyz (3 floats per vertex). *n_springs receives the number of directed
+ * springs S; a_ids, b_ids, ks, l0s (S each), spring_info (S: the springs listed at
+ * each vertex, vertex after vertex) and spring_ids (rows*cols + 1 offsets into
+ * spring_info) are written when non-null. Host memory; no device work. */
+int cm_ms_grid_springs(int32_t rows, int32_t cols, const float* xyz, float k_stretch,
+                       float k_shear, float k_bend, int32_t* a_ids, int32_t* b_ids, float* ks,
+                       float* l0s, int32_t* spring_info, int32_t* spring_ids,
+                       int32_t* n_springs);
+/* One step of the mass-spring cloth, ClothInternalDynamics
+ * (Physics/CudaDeformableBody.cpp:327-357): ClothEngineStep
+ * (Physics/CudaMassSpring.cu:22-88: force m g - k_damp (pos - prev) minus
+ * ks (d - l0 d/|d|) for every stretched spring listed at the vertex, Verlet step,
+ * prev = old pos, vel = (pos - prev) / dt) then adjust_iters passes of _AdjustSprings
+ * (:95-148; the reference runs 5 with max_deformation 0.1: a vertex whose spring
+ * that is not a bend spring (ks != k_bend) is stretched by more than
+ * max_deformation moves towards the other end by the excess, half of it when the
+ * other end is free; vel = (pos - prev) / dt). Fixed vertices (d_fixed != 0) keep
+ * position, previous position and velocity. Device buffers: d_pos, d_prev, d_vel
+ * float4 N (.w kept), d_mass float N, d_fixed N bytes, the spring arrays of
+ * cm_ms_grid_springs; gravity: 3 floats on the host; d_temp: N float4 of scratch.
+ * Springs with an end outside [0, n) are skipped. The reference updates positions
+ * in place while other threads read them; here every read of another vertex's
+ * position sees the value it had when its pass started. */
+int cm_ms_step(int32_t n, void* d_pos, void* d_prev, void* d_vel, const float* d_mass,
+               const uint8_t* d_fixed, const int32_t* d_a, const int32_t* d_b, const float* d_ks,
+               const float* d_l0, const int32_t* d_spring_info, const int32_t* d_spring_ids,
+               const float* gravity, float k_damp, float dt, float k_bend, int32_t adjust_iters,
+               float max_deformation, void* d_temp, void* stream);
+
+size_t cm_impulses_temp_bytes(int32_t n, int32_t n_contacts);
+/* Impulse response to a substep's contacts, SolveCollisions
+ * (Physics/CudaDeformableBody.cpp:525-548): CreateImpulses
+ * (Collision/CudaCollisionResponse.cu:11-231) at the start positions d_x and the
+ * velocities d_v (float4 N), for d_vf (n_vf) then d_ee (n_ee) device contacts, one
+ * impulse per contact vertex; each vertex's impulse is the mean of its impulses in
+ * the order of a stable sort by vertex id (AccumulateImpulses :233-256); then
+ * ApplyImpulses (:258-297): v += impulse / vertex_mass and p = x + v dt for every
+ * vertex with d_fixed == 0 (d_p float4 N; .w of v and p kept). stiffness is the
+ * reference's kStretch argument. A contact with a vertex id outside [0, n) gives
+ * no impulse. d_temp: cm_impulses_temp_bytes(n, n_vf + n_ee) bytes. */
+int cm_impulses(int32_t n, const void* d_x, void* d_v, void* d_p, const uint8_t* d_fixed,
+                const cm_contact* d_vf, int32_t n_vf, const cm_contact* d_ee, int32_t n_ee,
+                float stiffness, float vertex_mass, float dt, float thickness, void* d_temp,
+                size_t temp_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,14 @@ def lib():
     L.cm_lbvh_pairs_temp_bytes.argtypes = [i32, i32]
     L.cm_lbvh_pairs_temp_bytes.restype = C.c_size_t
     L.cm_lbvh_pairs.argtypes = [i32, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, C.c_size_t, vp]
+    L.cm_ms_grid_springs.argtypes = [i32, i32, vp, C.c_float, C.c_float, C.c_float, vp, vp, vp,
+                                     vp, vp, vp, C.POINTER(i32)]
+    L.cm_ms_step.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_float,
+                             C.c_float, C.c_float, i32, C.c_float, vp, vp]
+    L.cm_impulses_temp_bytes.argtypes = [i32, i32]
+    L.cm_impulses_temp_bytes.restype = C.c_size_t
+    L.cm_impulses.argtypes = [i32, vp, vp, vp, vp, vp, i32, vp, i32, C.c_float, C.c_float,
+                              C.c_float, C.c_float, vp, C.c_size_t, vp]
     L.cm_time_kernel.argtypes = [vp, i32, i32, C.POINTER(C.c_float)]
     L.cm_stage_times.argtypes = [vp, i32, vp]
     L.cm_set_stiffness.argtypes = [vp, vp, vp]
@@ -144,7 +152,8 @@ EXPORTS = (
     "cm_lbvh_sort",
     "cm_lbvh_build_temp_bytes", "cm_lbvh_build", "cm_lbvh_refit", "cm_lbvh_pairs_temp_bytes",
     "cm_lbvh_pairs", "cm_time_kernel", "cm_stage_times", "cm_set_stiffness", "cm_forward_ckpt",
-    "cm_adjoint", "cm_autotune", "cm_read_vertices",
+    "cm_adjoint", "cm_autotune", "cm_read_vertices", "cm_ms_grid_springs", "cm_ms_step",
+    "cm_impulses_temp_bytes", "cm_impulses",
 )
 # Rendering::VertexFormat as cm_read_vertices fills it (position, color, normal, uv)
 VERTEX_DTYPE = np.dtype([("position", "<f4", (3,)), ("color", "<f4", (4,)),

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libclothmi.so")
 SOURCES = ["k_prims.hip", "k_lbvh.hip", "k_narrow.hip", "k_pbd.hip", "cm_body.hip",
-           "cm_noalloc.hip"]
+           "cm_noalloc.hip", "k_springs.hip", "k_response.hip"]
 ARCH = os.environ.get("CM_OFFLOAD_ARCH", "gfx950")
 FLAGS = [
     "-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-ffp-contract=off",

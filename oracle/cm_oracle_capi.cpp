@@ -6,6 +6,7 @@
 #include <type_traits>
 
 #include "cm_oracle.hpp"
+#include "cm_oracle_alt.hpp"
 
 using namespace orc;
 
@@ -553,6 +554,49 @@ int orc_adjoint(void* hp, double* lx, double* lv, double* gk_dist, double* gk_be
     }
     return 0;
   });
+}
+
+// ---- dead paths (cm_oracle_alt.hpp), FP32 on 3N-float arrays ------------------
+int orc_ms_grid_springs(int rows, int cols, const float* xyz, float k_stretch, float k_shear,
+                        float k_bend, int32_t* a, int32_t* b, float* ks, float* l0,
+                        int32_t* info, int32_t* ids, int32_t* n_springs) {
+  const alt::Springs S = alt::grid_springs(rows, cols, xyz, k_stretch, k_shear, k_bend);
+  *n_springs = (int32_t)S.a.size();
+  for (size_t s = 0; s < S.a.size(); ++s) {
+    if (a) a[s] = S.a[s];
+    if (b) b[s] = S.b[s];
+    if (ks) ks[s] = S.ks[s];
+    if (l0) l0[s] = S.l0[s];
+    if (info) info[s] = S.info[s];
+  }
+  for (size_t v = 0; v < S.ids.size(); ++v)
+    if (ids) ids[v] = S.ids[v];
+  return 0;
+}
+
+int orc_ms_step(int n, float* pos, float* prev, float* vel, const float* mass,
+                const uint8_t* fixed, const int32_t* a, const int32_t* b, const float* ks,
+                const float* l0, const int32_t* info, const int32_t* ids, int n_springs,
+                const float* g, float k_damp, float dt, float k_bend, int adjust_iters,
+                float max_def) {
+  alt::Springs S;
+  S.a.assign(a, a + n_springs);
+  S.b.assign(b, b + n_springs);
+  S.ks.assign(ks, ks + n_springs);
+  S.l0.assign(l0, l0 + n_springs);
+  S.ids.assign(ids, ids + n + 1);
+  S.info.assign(info, info + ids[n]);
+  alt::ms_step<float>(n, pos, prev, vel, mass, fixed, S, g, k_damp, dt, k_bend, adjust_iters,
+                      max_def);
+  return 0;
+}
+
+int orc_impulses(int n, const float* x, float* v, float* p, const uint8_t* fixed,
+                 const cm_contact* vf, int nvf, const cm_contact* ee, int nee, float stiffness,
+                 float vertex_mass, float dt, float thickness) {
+  alt::impulses<float>(n, x, v, p, fixed, vf, nvf, ee, nee, stiffness, vertex_mass, dt,
+                       thickness);
+  return 0;
 }
 
 }  // extern "C"

@@ -81,6 +81,12 @@ def lib():
     L.orc_accumulate.argtypes = [i32, i64, vp, vp, vp, vp]
     L.orc_lra_vertex.argtypes = [vp, i32, vp, vp, vp, i32, vp, vp, vp]
     L.orc_damp.argtypes = [i32, i32, vp, vp, vp, C.c_float, C.c_float]
+    f32 = C.c_float
+    L.orc_ms_grid_springs.argtypes = [i32, i32, vp, f32, f32, f32, vp, vp, vp, vp, vp, vp,
+                                      C.POINTER(i32)]
+    L.orc_ms_step.argtypes = [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, vp, f32, f32,
+                              f32, i32, f32]
+    L.orc_impulses.argtypes = [i32, vp, vp, vp, vp, vp, i32, vp, i32, f32, f32, f32, f32]
     _lib = L
     return L
 
@@ -385,3 +391,52 @@ def damp(positions, velocities, masses, kg: float, kd: float, precision: int = 6
     m = np.ascontiguousarray(masses, dtype=np.float64)
     lib().orc_damp(precision, len(ps), _p(ps), _p(vs), _p(m), kg, kd)
     return vs
+
+
+# ---- dead paths of the reference's GPU cloth (cm_oracle_alt.hpp) ----------
+def ms_grid_springs(rows: int, cols: int, xyz, k_stretch: float, k_shear: float, k_bend: float):
+    """The grid spring network: dict of a, b, ks, l0, info, ids (numpy arrays)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    ns = C.c_int32()
+    L = lib()
+    L.orc_ms_grid_springs(rows, cols, _p(xyz), k_stretch, k_shear, k_bend, None, None, None, None,
+                          None, None, C.byref(ns))
+    S = ns.value
+    out = dict(a=np.zeros(S, np.int32), b=np.zeros(S, np.int32), ks=np.zeros(S, np.float32),
+               l0=np.zeros(S, np.float32), info=np.zeros(S, np.int32),
+               ids=np.zeros(rows * cols + 1, np.int32))
+    L.orc_ms_grid_springs(rows, cols, _p(xyz), k_stretch, k_shear, k_bend, _p(out["a"]),
+                          _p(out["b"]), _p(out["ks"]), _p(out["l0"]), _p(out["info"]),
+                          _p(out["ids"]), C.byref(ns))
+    return out
+
+
+def ms_step(pos, prev, vel, mass, fixed, springs: dict, gravity, k_damp: float, dt: float,
+            k_bend: float, adjust_iters: int = 5, max_deformation: float = 0.1):
+    """One mass-spring step (FP32); returns new (pos, prev, vel) as (N, 3) arrays."""
+    pos = np.ascontiguousarray(pos, dtype=np.float32).copy()
+    prev = np.ascontiguousarray(prev, dtype=np.float32).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float32).copy()
+    mass = np.ascontiguousarray(mass, dtype=np.float32)
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    sp = {k: np.ascontiguousarray(v) for k, v in springs.items()}
+    g = np.ascontiguousarray(gravity, dtype=np.float32)
+    lib().orc_ms_step(len(pos), _p(pos), _p(prev), _p(vel), _p(mass), _p(fixed), _p(sp["a"]),
+                      _p(sp["b"]), _p(sp["ks"]), _p(sp["l0"]), _p(sp["info"]), _p(sp["ids"]),
+                      len(sp["a"]), _p(g), k_damp, dt, k_bend, adjust_iters, max_deformation)
+    return pos, prev, vel
+
+
+def impulses(x, v, p, fixed, vf, ee, stiffness: float, vertex_mass: float, dt: float,
+             thickness: float):
+    """Impulse response (FP32) to the VF then EE contacts (CONTACT_DTYPE arrays);
+    returns new (v, p) as (N, 3) arrays."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    v = np.ascontiguousarray(v, dtype=np.float32).copy()
+    p = np.ascontiguousarray(p, dtype=np.float32).copy()
+    fixed = np.ascontiguousarray(fixed, dtype=np.uint8)
+    vf = np.ascontiguousarray(vf, dtype=CONTACT_DTYPE)
+    ee = np.ascontiguousarray(ee, dtype=CONTACT_DTYPE)
+    lib().orc_impulses(len(x), _p(x), _p(v), _p(p), _p(fixed), _p(vf), len(vf), _p(ee), len(ee),
+                       stiffness, vertex_mass, dt, thickness)
+    return v, p

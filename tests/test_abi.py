@@ -112,14 +112,15 @@ def test_flag_and_layout_constants_match_header():
 
 
 def test_integration_snippets_compile(tmp_path):
-    """The C++ blocks of INTEGRATION.md sections 2-4 (pair query on device buffers,
-    the batch, the adjoint) compile against include/clothmi.h as written."""
+    """The C++ blocks of INTEGRATION.md sections 2-5 (pair query on device buffers,
+    the batch, the adjoint, the dead paths) compile against include/clothmi.h as
+    written."""
     import shutil
     import subprocess
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
-    body = text[text.index("## 2."):text.index("## 5.")]
+    body = text[text.index("## 2."):text.index("## 6.")]
     blocks = re.findall(r"```cpp\n(.*?)```", body, flags=re.S)
-    assert len(blocks) == 3
+    assert len(blocks) == 4
     src = tmp_path / "snippets.cpp"
     src.write_text("\n".join(blocks))
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
