@@ -857,6 +857,12 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
+// More records than this and each is finished by one lane (k_traverse_lanes) instead
+// of a whole wave: a wave per record pays off for the long tail of a few heavy
+// leaves, while with many records the lanes' serial walks keep more of them going
+// at once.
+constexpr int kLaneModeMin = 16384;
+
 __global__ void __launch_bounds__(64) k_traverse_resume(int t, const float4* __restrict__ nodes,
                                                         const float4* __restrict__ amin,
                                                         const float4* __restrict__ amax,
@@ -876,6 +882,7 @@ __global__ void __launch_bounds__(64) k_traverse_resume(int t, const float4* __r
   const int nin = t - 1;
   const long long limit = 64ll * t + 4096;
   const int nres = min(hctl[0], rcap);
+  if (nres > kLaneModeMin) return;  // k_traverse_lanes takes them
   uint32_t wacc = 0;
   for (int rid = blockIdx.x; rid < nres; rid += gridDim.x) {
     const int* R = rec + (int64_t)rid * kRec;
@@ -1043,6 +1050,43 @@ __global__ void __launch_bounds__(64) k_traverse_resume(int t, const float4* __r
   // the workgroup's warnings in one atomic (the record count is cleared by the next
   // kernel on the stream, once every workgroup here has read it)
   if (lane == 0 && wacc) atomicOr(warn, wacc);
+}
+
+// Second pass when there are more than kLaneModeMin records: one lane per record
+// walks on serially from the record's state (next node, stack, pairs written,
+// visits), exactly as the first pass would have without a budget.
+__global__ void __launch_bounds__(kTravBlock) k_traverse_lanes(
+    int t, const float4* __restrict__ nodes, const float4* __restrict__ amin,
+    const float4* __restrict__ amax, int2* __restrict__ chunk, int stride,
+    int* __restrict__ counts, uint32_t* __restrict__ warn, int* __restrict__ tile_sum,
+    const int* __restrict__ rec, int rcap, const int* __restrict__ hctl,
+    int* __restrict__ spill) {
+  __shared__ int sstk[kLdsStack][kTravBlock];
+  const int nres = min(hctl[0], rcap);
+  if (nres <= kLaneModeMin) return;  // k_traverse_resume takes them
+  const int nin = t - 1;
+  const long long limit = 64ll * t + 4096;
+  uint32_t w = 0;
+  for (int rid = blockIdx.x * kTravBlock + threadIdx.x; rid < nres;
+       rid += gridDim.x * kTravBlock) {
+    const int* R = rec + (int64_t)rid * kRec;
+    const int leaf = R[0], top0 = R[2];
+    const int me = nin + leaf;
+    int* sp = spill + (int64_t)leaf * kSpill;
+    for (int k = 0; k < top0; ++k) {
+      if (k < kLdsStack) sstk[k][threadIdx.x] = R[6 + k];
+      else sp[k - kLdsStack] = R[6 + k];
+    }
+    DfsState st{R[3], top0, R[1], (long long)R[5]};
+    dfs_run<kTravBlock, kLdsStack>(nin, me, leaf, xyz(amin[me]), xyz(amax[me]), nodes, R[4],
+                                   chunk + (int64_t)leaf * stride, limit, LLONG_MAX, st, w, sstk,
+                                   threadIdx.x, sp);
+    counts[leaf] = st.ncol;
+    atomicAdd(&tile_sum[leaf / kTravBlock], st.ncol);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) w |= __shfl_xor(w, o, 64);
+  if ((threadIdx.x & 63) == 0 && w) atomicOr(warn, w);
 }
 
 __global__ void k_clear_records(int* __restrict__ hctl) {
@@ -1224,10 +1268,16 @@ static int launch_traverse(const LbvhBufs& b, int cap, int2* chunk, int* counts,
                      cap, chunk, counts, b.warn, b.ncloth ? b.slot_cloth : nullptr,
                      b.ncloth ? b.croot : nullptr, b.ccap, offsets, xchunk,
                      two ? b.trav_budget : 0, two ? b.trec : nullptr, b.rcap, b.hctl, b.spill);
-  if (two)
+  if (two) {  // one of the two finishes the records, the other returns at once
     hipLaunchKernelGGL(k_traverse_resume, dim3(std::min(b.rcap, kResumeBlocks)), dim3(64), 0, s,
                        t, b.nodes, b.amin, b.amax, chunk, cap, counts, b.warn, offsets, b.trec,
                        b.rcap, b.hctl);
+    if (b.rcap > kLaneModeMin)
+      hipLaunchKernelGGL(k_traverse_lanes,
+                         dim3(std::min((b.rcap + kTravBlock - 1) / kTravBlock, 8192)),
+                         dim3(kTravBlock), 0, s, t, b.nodes, b.amin, b.amax, chunk, cap, counts,
+                         b.warn, offsets, b.trec, b.rcap, b.hctl, b.spill);
+  }
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
