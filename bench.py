@@ -14,10 +14,11 @@ timed window (`value`). `full_run` gives the whole 1,200-substep drop's rate and
 `early_window` its first 20 substeps. With --steps 1200 or more the whole run is
 timed.
 
-Prints one JSON line. ``roofline`` covers the window's dominant kernel (the BVH
-traversal or the fused Jacobi iteration, whichever takes more time per substep)
-with its algorithmic bytes per launch over its average launch time measured live
-with HIP events on the body's stream; ``roofline_other`` gives the other one.
+Prints one JSON line. ``roofline`` covers the window's dominant kernel (the CCD
+plan pass, the BVH traversal or the fused Jacobi iteration, whichever takes the
+most time per substep) with its algorithmic bytes per launch over its average
+launch time measured live with HIP events on the body's stream;
+``roofline_other`` gives the other two.
 ``cpu_baseline`` times the FP32 CPU oracle seeded with the GPU's state entering the
 window, on a bounded sample of the same substeps on this host's cores.
 """
@@ -40,7 +41,7 @@ METRIC = "PBD substeps/s at 256^2 self-colliding cloth"
 
 FRAMES_SUBSTEPS = 1200  # 600 frames at 2 substeps per frame
 EARLY_WINDOW = 20
-KERNEL_SOLVE, KERNEL_TRAVERSE = 0, 2
+KERNEL_SOLVE, KERNEL_TRAVERSE, KERNEL_CCD_PLAN = 0, 2, 3
 WARN_CANDIDATE_CAP = 1 << 4
 
 
@@ -60,7 +61,8 @@ def _traffic_profiles(size: int) -> dict:
     by a plain copy's reading; (None, None) where no profile exists."""
     out = {}
     for key, name in (("traverse", "k_traverse_sphere_drop_256_traffic.json"),
-                      ("solve", "k_solve_sphere_drop_256_traffic.json")):
+                      ("solve", "k_solve_sphere_drop_256_traffic.json"),
+                      ("plan", "k_ccd_plan_sphere_drop_256_traffic.json")):
         tf = os.path.join(ROOT, "profiles", name)
         if size == 256 and os.path.exists(tf):
             tj = json.load(open(tf))
@@ -210,9 +212,13 @@ def main():
     counts_after = body.counts()
     iters = scene["params"]["solver_iterations"]
     contacts = counts_after["vf"] + counts_after["ee"]
-    # live HIP-event timing of the window's two largest kernels on its final state:
-    # the BVH traversal (one launch per substep) and the Jacobi iteration (15 per substep)
+    # live HIP-event timing of the window's three largest kernels on its final state:
+    # the CCD plan pass and the BVH traversal (one launch each per substep) and the
+    # Jacobi iteration (15 per substep)
     avg = C.c_float()
+    if lib().cm_time_kernel(body.h, KERNEL_CCD_PLAN, args.kernel_reps, C.byref(avg)):
+        raise RuntimeError("cm_time_kernel(ccd plan) failed")
+    plan_ms = float(avg.value)
     if lib().cm_time_kernel(body.h, KERNEL_TRAVERSE, args.kernel_reps, C.byref(avg)):
         raise RuntimeError("cm_time_kernel(traverse) failed")
     trav_ms = float(avg.value)
@@ -224,9 +230,16 @@ def main():
     trav_bytes = 88 * t + 8 * counts_after["pairs"]
     trav_gbs = trav_bytes / (trav_ms * 1e-3) / 1e9
     solve_gbs = per_iter / (solve_ms * 1e-3) / 1e9
+    # the CCD plan pass: each candidate's 16-byte id record read and its near flag
+    # written (positions and velocities are L2 hits; the near candidates' 24-byte plans,
+    # about 7% of the candidates, are left out, so this is a lower bound)
+    ncand = counts_after["vf_cand"] + counts_after["ee_cand"]
+    plan_bytes = 17 * ncand
+    plan_gbs = plan_bytes / (plan_ms * 1e-3) / 1e9
     # the dominant kernel of the window by time per substep
     trav_share, solve_share = trav_ms, solve_ms * iters
-    dominant = "traverse" if trav_share >= solve_share else "solve"
+    shares = dict(plan=plan_ms, traverse=trav_share, solve=solve_share)
+    dominant = max(shares, key=shares.get)
     prof = _traffic_profiles(args.size)
     roof_trav = dict(bound="hbm", achieved=trav_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
                      frac=trav_gbs / HBM_PEAK_GBS, traffic=prof["traverse"][0],
@@ -240,7 +253,15 @@ def main():
                       kernel="k_solve (fused Jacobi iteration, 15 launches per substep)",
                       bytes_per_launch=per_iter, avg_launch_ms=solve_ms,
                       ms_per_substep=solve_share)
-    roof = roof_trav if dominant == "traverse" else roof_solve
+    roof_plan = dict(bound="hbm", achieved=plan_gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=plan_gbs / HBM_PEAK_GBS, traffic=prof["plan"][0],
+                     traffic_source=prof["plan"][1],
+                     kernel="k_ccd_plan (CCD plan, secant roots and near test, one launch "
+                            "per substep; VALU-bound by the secant searches' divergence)",
+                     bytes_per_launch=plan_bytes, avg_launch_ms=plan_ms,
+                     ms_per_substep=plan_ms, candidates=ncand)
+    roofs = dict(plan=roof_plan, traverse=roof_trav, solve=roof_solve)
+    roof = roofs[dominant]
     full_s = t_early + t_rest + t_win
     out = dict(
         metric=METRIC,
@@ -261,7 +282,7 @@ def main():
                     timed_window=f"substeps {preroll + 1}-{run_len} of {run_len}",
                     parallelism="single cloth, one GPU"),
         roofline=roof,
-        roofline_other=roof_solve if dominant == "traverse" else roof_trav,
+        roofline_other=[r for k, r in roofs.items() if k != dominant],
         vertex_updates_per_s=n * 1e3 / ms_per_step,
         substep_roofline_frac=per_substep / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
         full_run=dict(substeps=run_len, seconds=full_s, substeps_per_s=run_len / full_s,

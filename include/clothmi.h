@@ -257,6 +257,10 @@ int cm_query_contacts(cm_body* body, cm_contact* vf, int32_t cap_vf, int32_t* n_
 /* the BVH traversal of the current tree (rewrites the same per-leaf pair chunk;
  * the compacted pair list is unchanged) */
 #define CM_KERNEL_TRAVERSE 2
+/* the CCD plan pass (cubic, secant roots, near test) over the last substep's
+ * candidates, with the current positions and velocities (rewrites its own plan
+ * words and near flags; the contact lists are unchanged) */
+#define CM_KERNEL_CCD_PLAN 3
 int cm_time_kernel(cm_body* body, int32_t which, int32_t reps, float* avg_ms);
 
 /* Per-stage timing of `substeps` substeps run eagerly on the body's stream with a

@@ -1201,12 +1201,13 @@ int cm_time_kernel(cm_body* b, int32_t which, int32_t reps, float* avg_ms) {
   int rc = body_check(b);
   if (rc) return rc;
   if (reps < 1 || !avg_ms ||
-      (which != CM_KERNEL_SOLVE && which != CM_KERNEL_SUBSTEP && which != CM_KERNEL_TRAVERSE)) {
+      (which != CM_KERNEL_SOLVE && which != CM_KERNEL_SUBSTEP && which != CM_KERNEL_TRAVERSE &&
+       which != CM_KERNEL_CCD_PLAN)) {
     set_error("cm_time_kernel: invalid argument");
     return CM_ERR_ARG;
   }
   const bool coll = (b->prm.flags & CM_F_SELF_COLLISION) && b->t >= 2;
-  if (which == CM_KERNEL_TRAVERSE && !coll) {
+  if ((which == CM_KERNEL_TRAVERSE || which == CM_KERNEL_CCD_PLAN) && !coll) {
     set_error("cm_time_kernel: the body has no self-collision tree");
     return CM_ERR_STATE;
   }
@@ -1221,6 +1222,10 @@ int cm_time_kernel(cm_body* b, int32_t which, int32_t reps, float* avg_ms) {
     NarrowBufs nb = b->nb;
     if (!coll) nb.dprefix = nullptr;
     rc = launch_solve_only(pbd_bufs(b), pbd_params(b), nb, reps, b->stream);
+  } else if (which == CM_KERNEL_CCD_PLAN) {
+    for (int r = 0; r < reps && !rc; ++r)
+      rc = launch_ccd_plan_only(b->nb, b->x, b->v, b->prm.thickness, b->prm.timestep,
+                                (b->prm.flags & CM_F_DCD) ? 1 : 0, b->stream);
   } else if (which == CM_KERNEL_TRAVERSE) {
     for (int r = 0; r < reps && !rc; ++r)
       rc = launch_traverse_only(b->lb, b->cap, b->chunk, b->leaf_cnt, b->leaf_off, b->stream);

@@ -176,6 +176,9 @@ struct NarrowBufs {
   void* scan_temp = nullptr;
 };
 // dcd: one proximity test per candidate at x instead of the CCD (CM_F_DCD)
+// the CCD plan pass alone over the current candidate lists (kernel timing)
+int launch_ccd_plan_only(const NarrowBufs& nb, const float4* x, const float4* v, float th,
+                         float dt, int dcd, hipStream_t s);
 int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const float4* p,
                   const float4* v, float th, float dt, int ee_on, int dcd, hipStream_t s);
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s);

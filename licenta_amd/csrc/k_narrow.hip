@@ -907,6 +907,16 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   return CM_OK;
 }
 
+int launch_ccd_plan_only(const NarrowBufs& nb, const float4* x, const float4* v, float th,
+                         float dt, int dcd, hipStream_t s) {
+  const int64_t ccap = nb.vf_cap + nb.ee_cap;
+  hipLaunchKernelGGL(k_ccd_plan, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.cand_vf,
+                     nb.cand_ee, nb.d_ncand_vf, nb.d_ncand_ee, nb.vf_cap, nb.ee_cap, x, v, th, dt,
+                     dcd, nb.cplan, nb.ccp, nb.croot, nb.d_ntotal, nb.hit_flag, nb.tile_sum);
+  CM_LAUNCH_CHECK();
+  return CM_OK;
+}
+
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
   // dcount[0, n) is zero here: cleared at creation, then by every k_dyn_fill
   const int gc = grid_for(nb.vf_out_cap + nb.ee_out_cap);

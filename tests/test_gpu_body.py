@@ -437,3 +437,46 @@ def test_high_valence_fan_slot_parallel_solve(oracle_mod):
     inv_mass = np.full(len(xyz), 1000.0, np.float32)
     sc = configs._scene(params, xyz, tri, inv_mass, name="fan")
     res, w = run_scene(oracle_mod, sc, 3)
+
+
+def test_kernel_timing_leaves_the_step_unchanged():
+    """cm_time_kernel on the traversal and on the CCD plan pass (CM_KERNEL_TRAVERSE,
+    CM_KERNEL_CCD_PLAN) re-runs them on the last substep's tree and candidates: the
+    contact lists and pairs stay as they were, and the following substep equals that
+    of a twin body that was not timed, bit for bit."""
+    import ctypes as C
+
+    from licenta_amd._abi import lib
+    from licenta_amd.body import ClothBody
+    n = 24
+    sc = configs.config_scene(0, n)
+    xyz = sc["xyz"].reshape(n, n, 3).copy()
+    h = n // 2
+    for i in range(h, n):
+        xyz[i, :, 0] = xyz[2 * h - 1 - i, :, 0] + 0.002
+        xyz[i, :, 1] = 0.003 + 0.0005 * (i - h)
+    sc["xyz"] = xyz.reshape(-1, 3).astype(np.float32)
+    sc["pinned"] = None
+    sc["inv_mass"] = np.full(n * n, 1000.0, np.float32)
+    sc = dict(sc, params=dict(sc["params"], thickness=0.004))
+    v = np.zeros((n * n, 3), np.float32)
+    v.reshape(n, n, 3)[h:, :, 1] = -0.3
+    bodies = [ClothBody.from_scene(sc) for _ in range(2)]
+    for b in bodies:
+        b.set_state(sc["xyz"], v)
+        b.fixed_update(1)
+    vf0, ee0 = bodies[0].contacts()
+    pairs0 = bodies[0].pairs()
+    assert len(vf0) + len(ee0) > 0
+    avg = C.c_float()
+    for which in (2, 3):
+        assert lib().cm_time_kernel(bodies[0].h, which, 3, C.byref(avg)) == 0
+        assert avg.value > 0
+    vf1, ee1 = bodies[0].contacts()
+    assert np.array_equal(vf1, vf0) and np.array_equal(ee1, ee0)
+    assert np.array_equal(bodies[0].pairs(), pairs0)
+    for b in bodies:
+        b.fixed_update(1)
+    assert np.array_equal(bodies[0].positions(), bodies[1].positions())
+    for b in bodies:
+        b.close()
