@@ -871,6 +871,21 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
       (rc = dalloc(b, &nb.rec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
       (rc = dalloc(b, &nb.dlong, (size_t)n)))
     return fail(rc);
+  // The sorted record CSR is set up when the records can exceed 32 Mi (4 per contact),
+  // and runs in substeps with more than 2 Mi records (CLOTHMI_SORT_CSR overrides the
+  // threshold; 0: never)
+  {
+    const int64_t rcap = 4 * (nb.vf_out_cap + nb.ee_out_cap);
+    int64_t sort_min = rcap >= ((int64_t)32 << 20) ? ((int64_t)2 << 20) : 0;
+    if (const char* e = std::getenv("CLOTHMI_SORT_CSR")) sort_min = std::max(0, std::atoi(e));
+    if (sort_min > 0 && rcap < ((int64_t)1 << 31)) {
+      nb.sort_min = (int)sort_min;
+      if ((rc = dalloc(b, &nb.skey, 2 * (size_t)rcap)) ||
+          (rc = dalloc(b, (char**)&nb.sort_temp, radix_sort_dev_temp_bytes())))
+        return fail(rc);
+      CM_HIP(hipMemset(nb.sort_temp, 0, radix_sort_dev_temp_bytes()));
+    }
+  }
   CM_HIP(hipMemset(nb.dprefix, 0, sizeof(int) * ((size_t)n + 1)));
   CM_HIP(hipMemset(nb.dcount, 0, sizeof(int) * ((size_t)n + 1)));  // kept zero between substeps
   if (params->flags & CM_F_SPHERE) {

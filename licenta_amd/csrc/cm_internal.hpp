@@ -67,6 +67,16 @@ int scan_tiles(const int* d_items, int per_tile, int ntile_max, const int* sums,
 size_t radix_sort_temp_bytes(int n);
 int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s);
 int radix_sort_clear(void* temp, int n, hipStream_t s);  // before the first sort on a temp
+// Stable LSD sort of the first min(*d_n, n_max) 32-bit keys (key_bits <= 32) with
+// their values, with a launch shape independent of the count (graph-capturable). The
+// result ends in keys_alt / vals_alt when *in_alt is set (an odd number of passes),
+// else in keys / vals. temp: radix_sort_dev_temp_bytes(), cleared once before its
+// first sort by radix_sort_dev_clear.
+size_t radix_sort_dev_temp_bytes();
+int radix_sort_dev_clear(void* temp, hipStream_t s);
+int radix_sort_dev(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                   int n_max, const int* d_n, int key_bits, void* temp, int* in_alt,
+                   hipStream_t s);
 int fill_i32(int* p, int v, int64_t n, hipStream_t s);
 int fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 
@@ -171,7 +181,13 @@ struct NarrowBufs {
   uint32_t* dtmp = nullptr;  // same size: staging for very long record lists
   RecPayload* rec = nullptr;  // same size: record payloads in drec order
   int* dlong = nullptr;      // n: vertices whose record lists are ordered by a workgroup
-  int* d_nrec = nullptr;
+  // sorted CSR (radix sort of the records by vertex id) for substeps with more than
+  // sort_min records; sort_min = 0: never. skey: record keys then alternate keys
+  // (2 * 4 * (vf_out_cap + ee_out_cap)), sort_temp: radix_sort_dev_temp_bytes()
+  int sort_min = 0;
+  uint32_t* skey = nullptr;
+  void* sort_temp = nullptr;
+  int* d_nrec = nullptr;     // records the sorted CSR orders this substep, 0 when it is off
   uint32_t* warn = nullptr;
   void* scan_temp = nullptr;
 };

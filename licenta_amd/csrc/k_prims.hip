@@ -255,6 +255,111 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(
   }
 }
 
+// ---- radix sort of 32-bit keys with a device-side count --------------------------
+// A fixed grid of kDevSortBlocks workgroups; workgroup b takes the contiguous run of
+// tiles [b * tpb, (b + 1) * tpb) of the n = min(*d_n, n_max) keys (tpb = ceil(tiles /
+// grid)), so one histogram per workgroup keeps the sort stable and the launch shape
+// does not depend on n.
+constexpr int kDevSortBlocks = 1024;
+
+__device__ __forceinline__ void dev_sort_range(const int* d_n, int n_max, int& n, int& t0,
+                                               int& t1) {
+  n = min(*d_n, n_max);
+  const int ntiles = (n + kSortTile - 1) / kSortTile;
+  const int tpb = (ntiles + gridDim.x - 1) / gridDim.x;
+  t0 = min(ntiles, blockIdx.x * tpb);
+  t1 = min(ntiles, t0 + tpb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_radix_hist_dev(const uint32_t* __restrict__ keys,
+                                                           const int* __restrict__ d_n, int n_max,
+                                                           int shift, int* __restrict__ hist) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int n, t0, t1;
+  dev_sort_range(d_n, n_max, n, t0, t1);
+  for (int tile = t0; tile < t1; ++tile) {
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+      const int i = tile * kSortTile + k * kBlock + threadIdx.x;
+      if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1);
+    }
+  }
+  __syncthreads();
+  hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// k_radix_scatter over the workgroup's run of tiles, each digit's running base carried
+// from tile to tile (thread d owns digit d of carry, wbase and the count sums)
+__global__ void __launch_bounds__(kBlock) k_radix_scatter_dev(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, const int* __restrict__ d_n,
+    int n_max, int shift, const int* __restrict__ offsets) {
+  __shared__ int cnt[kWaves][256];
+  __shared__ int wbase[kWaves][256];
+  __shared__ int carry[256];
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  int n, t0, t1;
+  dev_sort_range(d_n, n_max, n, t0, t1);
+  carry[threadIdx.x] = 0;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (int tile = t0; tile < t1; ++tile) {
+    for (int d = threadIdx.x; d < kWaves * 256; d += kBlock) (&cnt[0][0])[d] = 0;
+    __syncthreads();
+    const int base = tile * kSortTile + wid * 64 * kSortItems;
+    uint32_t key[kSortItems];
+    uint32_t val[kSortItems];
+    int rank[kSortItems];
+#pragma unroll
+    for (int c = 0; c < kSortItems; ++c) {
+      const int i = base + c * 64 + lane;
+      const bool valid = i < n;
+      key[c] = valid ? kin[i] : 0u;
+      val[c] = valid ? vin[i] : 0u;
+      const uint32_t d = (key[c] >> shift) & 255u;
+      uint64_t m = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+      }
+      int r = 0;
+      if (valid) {
+        const int before = __popcll(m & lt);
+        const int old = cnt[wid][d];
+        r = old + before;
+        if (before == 0) cnt[wid][d] = old + __popcll(m);
+      }
+      rank[c] = r;
+    }
+    __syncthreads();
+    {
+      const int d = threadIdx.x;  // kBlock == 256 digits
+      int run = carry[d];
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        wbase[w][d] = run;
+        run += cnt[w][d];
+      }
+      carry[d] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kSortItems; ++c) {
+      const int i = base + c * 64 + lane;
+      if (i < n) {
+        const uint32_t d = (key[c] >> shift) & 255u;
+        const int pos = offsets[d * gridDim.x + blockIdx.x] + wbase[wid][d] + rank[c];
+        kout[pos] = key[c];
+        vout[pos] = val[c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---- scan of per-tile sums ----------------------------------------------------------
 // Kernels that produce counts in tiles of consecutive items also write each tile's sum;
 // this one workgroup scans the sums into tile prefixes, and the consuming kernel
@@ -406,6 +511,38 @@ int radix_sort_clear(void* temp, int n, hipStream_t s) {
   char* tp = (char*)temp + align_up(sizeof(uint64_t) * (size_t)n) +
              align_up(sizeof(uint32_t) * (size_t)n) + align_up(sizeof(int) * 256 * (size_t)nb);
   return scan_temp_clear(tp, 256 * nb, s);
+}
+
+size_t radix_sort_dev_temp_bytes() {
+  return align_up(sizeof(int) * 256 * kDevSortBlocks) + scan_temp_bytes(256 * kDevSortBlocks);
+}
+
+int radix_sort_dev_clear(void* temp, hipStream_t s) {
+  return scan_temp_clear((char*)temp + align_up(sizeof(int) * 256 * kDevSortBlocks),
+                         256 * kDevSortBlocks, s);
+}
+
+int radix_sort_dev(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt,
+                   int n_max, const int* d_n, int key_bits, void* temp, int* in_alt,
+                   hipStream_t s) {
+  int* hist = (int*)temp;
+  void* stemp = (char*)temp + align_up(sizeof(int) * 256 * kDevSortBlocks);
+  const int passes = std::min(4, (key_bits + 7) / 8);
+  uint32_t *ka = keys, *va = vals, *kb = keys_alt, *vb = vals_alt;
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = 8 * pass;
+    hipLaunchKernelGGL(k_radix_hist_dev, dim3(kDevSortBlocks), dim3(kBlock), 0, s, ka, d_n, n_max,
+                       shift, hist);
+    int rc = scan_exclusive(hist, hist, 256 * kDevSortBlocks, nullptr, nullptr, stemp, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_radix_scatter_dev, dim3(kDevSortBlocks), dim3(kBlock), 0, s, ka, va, kb,
+                       vb, d_n, n_max, shift, hist);
+    CM_LAUNCH_CHECK();
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  *in_alt = passes & 1;
+  return CM_OK;
 }
 
 int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s) {
