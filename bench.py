@@ -168,7 +168,7 @@ def main():
 
     from licenta_amd import configs
     from licenta_amd._abi import lib
-    from licenta_amd.body import ClothBody
+    from licenta_amd.body import ClothBody, warning_names
     import ctypes as C
 
     scene = configs.config_scene(1, args.size)
@@ -201,6 +201,7 @@ def main():
     ms_per_step = t_win * 1e3 / args.steps
     counts = body.counts()
     warn = body.warnings()
+    tree = body.tree_coverage()
     # per-stage split of the window's state (eager launches between HIP events, 10 more
     # substeps)
     stage_names = ("predict", "triangles", "sort", "tree", "pairs", "narrow", "solver", "integrate")
@@ -294,7 +295,7 @@ def main():
         window_start=dict(pairs=counts_pre["pairs"], vf=counts_pre["vf"], ee=counts_pre["ee"],
                           warnings=warn_pre),
         last_substep=dict(pairs=counts["pairs"], vf=counts["vf"], ee=counts["ee"],
-                          warnings=warn),
+                          warnings=warn, warning_names=warning_names(warn), tree=tree),
         autotuned=tuned,
         stage_ms_after_run=stage_ms,
     )
@@ -377,14 +378,9 @@ def batched_leg(ncloth: int, steps: int, warmup: int = 3) -> dict:
                                    "narrow-phase work and the ratio is not comparable"))
 
 
-def large_leg(size: int, steps: int, warmup: int = 3) -> dict:
-    """The crumpled large cloth: 1024x1024 vertices, spacing 1 mm, y displaced by
-    8 random sinusoids (seed 4), thickness 0.5 mm, 20 iterations, 1 substep per
-    frame at dt 1/60 s, 200 frames, full Morton/sort/LBVH rebuild every substep."""
-    from licenta_amd import configs
-    from licenta_amd.body import ClothBody
+def _large_run(sc: dict, steps: int, warmup: int) -> dict:
+    from licenta_amd.body import ClothBody, warning_names
 
-    sc = configs.config_scene(4, size)
     t0 = time.perf_counter()
     body = ClothBody.from_scene(sc)
     t_create = time.perf_counter() - t0
@@ -396,13 +392,36 @@ def large_leg(size: int, steps: int, warmup: int = 3) -> dict:
     dt = time.perf_counter() - t0
     c = body.counts()
     w = body.warnings()
+    tree = body.tree_coverage()
     body.close()
+    return dict(c=c, dt=dt, create=t_create,
+                last=dict(pairs=c["pairs"], vf=c["vf"], ee=c["ee"], warnings=w,
+                          warning_names=warning_names(w), tree=tree))
+
+
+def large_leg(size: int, steps: int, warmup: int = 3) -> dict:
+    """The crumpled large cloth: 1024x1024 vertices, spacing 1 mm, y displaced by
+    8 random sinusoids (seed 4), thickness 0.5 mm, 20 iterations, 1 substep per
+    frame at dt 1/60 s, 200 frames, full Morton/sort/LBVH rebuild every substep.
+    The reference's split (midpoint on tied 30-bit codes) leaves leaves out of the
+    tree on this grid; the same run with the index-augmented Karras split is
+    reported beside it (corrected_split)."""
+    from licenta_amd import configs
+
+    sc = configs.config_scene(4, size)
+    r = _large_run(sc, steps, warmup)
+    c, dt = r["c"], r["dt"]
+    params = dict(sc["params"], ref_compat=sc["params"]["ref_compat"] & ~configs.RC_MIDPOINT_SPLIT)
+    k = _large_run(dict(sc, params=params), steps, warmup)
     return dict(workload=f"crumpled_{size}x{size}", vertices=c["n"], triangles=c["t"],
                 distance_constraints=c["s"], bend_constraints=c["b"], iterations=20,
                 steps=steps, warmup=warmup, substeps_per_s=steps / dt,
                 vertex_updates_per_s=c["n"] * steps / dt, ms_per_step=dt * 1e3 / steps,
-                create_s=t_create,
-                last_substep=dict(pairs=c["pairs"], vf=c["vf"], ee=c["ee"], warnings=w))
+                create_s=r["create"], last_substep=r["last"],
+                corrected_split=dict(substeps_per_s=steps / k["dt"],
+                                     ms_per_step=k["dt"] * 1e3 / steps, last_substep=k["last"],
+                                     note="index-augmented Karras split (CM_RC_MIDPOINT_SPLIT "
+                                          "cleared): every leaf in the tree"))
 
 
 def _time_rollout(body, substeps, every, x0, v0, tri0, lx_of, reps=3):

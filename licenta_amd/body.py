@@ -33,6 +33,40 @@ WARN_NAMES = {1: "stack_overflow", 2: "traversal_limit", 4: "broken_tree", 8: "p
               16: "candidate_cap", 32: "refit_limit"}
 
 
+def warning_names(w: int) -> list:
+    """The CM_WARN_* bits of a warning word, by name."""
+    return [name for bit, name in sorted(WARN_NAMES.items()) if w & bit]
+
+
+def leaf_reach(lefts, rights) -> dict:
+    """Coverage of a Karras tree over T leaves: internal nodes 0..T-2 with children
+    lefts/rights, a child >= T-1 being leaf child - (T-1), the root node 0. With the
+    midpoint split on tied Morton codes (CM_RC_MIDPOINT_SPLIT) two internal nodes can
+    take the same child, and the leaves of the subtrees no path from the root enters
+    are never tested for collisions. Returns the leaf count, the leaves reached from
+    the root, the leaves not reached, and the nodes listed as a child more than once."""
+    lefts = np.asarray(lefts, np.int64)
+    rights = np.asarray(rights, np.int64)
+    nin = len(lefts)
+    t = nin + 1
+    if nin == 0:
+        return dict(leaves=t, reached=t, unreachable_leaves=0, shared_nodes=0)
+    ch_all = np.concatenate([lefts, rights])
+    shared = int((np.bincount(ch_all, minlength=2 * t - 1) > 1).sum())
+    seen_int = np.zeros(nin, bool)
+    seen_leaf = np.zeros(t, bool)
+    seen_int[0] = True
+    frontier = np.array([0], np.int64)
+    while frontier.size:
+        ch = np.concatenate([lefts[frontier], rights[frontier]])
+        seen_leaf[ch[ch >= nin] - nin] = True
+        inner = np.unique(ch[ch < nin])
+        frontier = inner[~seen_int[inner]]
+        seen_int[frontier] = True
+    reached = int(seen_leaf.sum())
+    return dict(leaves=t, reached=reached, unreachable_leaves=t - reached, shared_nodes=shared)
+
+
 class ClothBody:
     def __init__(self, xyz, tri, params: dict, fixed_verts=None, inv_mass=None, dist_pairs=None,
                  k_dist=None, k_bend=None):
@@ -168,6 +202,10 @@ class ClothBody:
         w = C.c_uint32()
         check(lib().cm_get_warnings(self.h, C.byref(w)))
         return int(w.value)
+
+    def tree_coverage(self) -> dict:
+        """leaf_reach over the tree of the last substep's build."""
+        return leaf_reach(self.buffer("lefts"), self.buffer("rights"))
 
     def buffer(self, name: str) -> np.ndarray:
         L = lib()
