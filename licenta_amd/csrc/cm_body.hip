@@ -871,15 +871,15 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
       (rc = dalloc(b, &nb.rec, 4 * (size_t)(nb.vf_out_cap + nb.ee_out_cap))) ||
       (rc = dalloc(b, &nb.dlong, (size_t)n)))
     return fail(rc);
-  // The sorted record CSR is set up when the records can exceed 32 Mi (4 per contact),
-  // and runs in substeps with more than 2 Mi records (CLOTHMI_SORT_CSR overrides the
-  // threshold; 0: never)
+  // The contact-record CSR comes from one radix sort of the records by vertex id;
+  // CLOTHMI_SORT_CSR=0 selects per-vertex counts with atomic placement instead (the
+  // same lists; the sort needs record capacities below 2^31)
   {
     const int64_t rcap = 4 * (nb.vf_out_cap + nb.ee_out_cap);
-    int64_t sort_min = rcap >= ((int64_t)32 << 20) ? ((int64_t)2 << 20) : 0;
-    if (const char* e = std::getenv("CLOTHMI_SORT_CSR")) sort_min = std::max(0, std::atoi(e));
-    if (sort_min > 0 && rcap < ((int64_t)1 << 31)) {
-      nb.sort_min = (int)sort_min;
+    bool sorted = rcap < ((int64_t)1 << 31);
+    if (const char* e = std::getenv("CLOTHMI_SORT_CSR")) sorted = sorted && std::atoi(e) != 0;
+    if (sorted) {
+      nb.sorted_csr = true;
       if ((rc = dalloc(b, &nb.skey, 2 * (size_t)rcap)) ||
           (rc = dalloc(b, (char**)&nb.sort_temp, radix_sort_dev_temp_bytes())))
         return fail(rc);

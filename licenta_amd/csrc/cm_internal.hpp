@@ -181,13 +181,14 @@ struct NarrowBufs {
   uint32_t* dtmp = nullptr;  // same size: staging for very long record lists
   RecPayload* rec = nullptr;  // same size: record payloads in drec order
   int* dlong = nullptr;      // n: vertices whose record lists are ordered by a workgroup
-  // sorted CSR (radix sort of the records by vertex id) for substeps with more than
-  // sort_min records; sort_min = 0: never. skey: record keys then alternate keys
-  // (2 * 4 * (vf_out_cap + ee_out_cap)), sort_temp: radix_sort_dev_temp_bytes()
-  int sort_min = 0;
+  // sorted_csr: the CSR comes from one radix sort of the records by vertex id
+  // (else from per-vertex counts and atomic placement). skey: record keys then
+  // alternate keys (2 * 4 * (vf_out_cap + ee_out_cap)), sort_temp:
+  // radix_sort_dev_temp_bytes()
+  bool sorted_csr = false;
   uint32_t* skey = nullptr;
   void* sort_temp = nullptr;
-  int* d_nrec = nullptr;     // records the sorted CSR orders this substep, 0 when it is off
+  int* d_nrec = nullptr;     // records of the substep (sorted CSR)
   uint32_t* warn = nullptr;
   void* scan_temp = nullptr;
 };

@@ -704,20 +704,13 @@ __global__ void __launch_bounds__(kBlock) k_ccd_write(
   if (over) atomicOr(warn, (uint32_t)CM_WARN_CANDIDATE_CAP);
 }
 
-// the sorted CSR takes the substep's records when there are more than sort_min
-__device__ __forceinline__ bool sorted_csr(int nvf, int nee, int sort_min) {
-  return sort_min > 0 && 4 * (int64_t)(nvf + nee) > sort_min;
-}
-
 __global__ void __launch_bounds__(kBlock) k_dyn_count(const int4* __restrict__ vfv,
                                                       const int4* __restrict__ eev,
                                                       const int* __restrict__ d_nvf,
                                                       const int* __restrict__ d_nee,
-                                                      int* __restrict__ dcount, int n,
-                                                      int sort_min) {
+                                                      int* __restrict__ dcount, int n) {
   const int nvf = *d_nvf, nee = *d_nee;
   if (blockIdx.x == 0 && threadIdx.x == 0) dcount[n] = 0;  // long-list counter of k_dyn_sort
-  if (sorted_csr(nvf, nee, sort_min)) return;
   for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
     const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
     atomicAdd(&dcount[q.x], 1);
@@ -733,12 +726,10 @@ __global__ void __launch_bounds__(kBlock) k_dyn_fill(const int4* __restrict__ vf
                                                      const int* __restrict__ d_nee,
                                                      const int* __restrict__ prefix,
                                                      int* __restrict__ dcount,
-                                                     uint32_t* __restrict__ drec,
-                                                     int sort_min) {
+                                                     uint32_t* __restrict__ drec) {
   // each vertex's count runs down to zero as its records are placed (in any order:
   // k_dyn_sort orders them), which leaves the counts cleared for the next substep
   const int nvf = *d_nvf, nee = *d_nee;
-  if (sorted_csr(nvf, nee, sort_min)) return;
   for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
     const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
     const int vv[4] = {q.x, q.y, q.z, q.w};
@@ -779,13 +770,11 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort(int n, const int* __restric
                                                      const float4* __restrict__ vf_w,
                                                      const int4* __restrict__ ee_v,
                                                      const float4* __restrict__ ee_w,
-                                                     RecPayload* __restrict__ rec,
-                                                     const int* __restrict__ d_nee,
-                                                     int sort_min) {
+                                                     RecPayload* __restrict__ rec) {
   __shared__ uint32_t sk[kShortList][kBlock];  // column per thread: no bank conflicts
   const int tid = threadIdx.x;
   const int v = blockIdx.x * kBlock + tid;
-  if (v >= n || sorted_csr(*d_nvf, *d_nee, sort_min)) return;
+  if (v >= n) return;
   const int b = prefix[v], e = prefix[v + 1];
   const int k = e - b;
   if (k > kShortList) {
@@ -863,14 +852,12 @@ __global__ void __launch_bounds__(kBlock) k_dyn_sort_long(const int* __restrict_
 __global__ void __launch_bounds__(kBlock) k_rec_keys(const int4* __restrict__ vfv,
                                                      const int4* __restrict__ eev,
                                                      const int* __restrict__ d_nvf,
-                                                     const int* __restrict__ d_nee, int sort_min,
+                                                     const int* __restrict__ d_nee,
                                                      uint4* __restrict__ keys,
                                                      uint4* __restrict__ vals,
                                                      int* __restrict__ d_nrec) {
   const int nvf = *d_nvf, nee = *d_nee;
-  const bool on = sorted_csr(nvf, nee, sort_min);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *d_nrec = on ? 4 * (nvf + nee) : 0;
-  if (!on) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d_nrec = 4 * (nvf + nee);
   for (int c = blockIdx.x * kBlock + threadIdx.x; c < nvf + nee; c += gridDim.x * kBlock) {
     const int4 q = c < nvf ? vfv[c] : eev[c - nvf];
     keys[c] = make_uint4((uint32_t)q.x, (uint32_t)q.y, (uint32_t)q.z, (uint32_t)q.w);
@@ -879,39 +866,37 @@ __global__ void __launch_bounds__(kBlock) k_rec_keys(const int4* __restrict__ vf
   }
 }
 
-// dprefix[v] = the number of sorted keys below v, for v in [0, n]
-__global__ void __launch_bounds__(kBlock) k_rec_prefix(int n, const uint32_t* __restrict__ keys,
-                                                       const int* __restrict__ d_nrec,
-                                                       int* __restrict__ dprefix) {
-  const int R = *d_nrec;
-  if (R == 0) return;
-  for (int v = blockIdx.x * kBlock + threadIdx.x; v <= n; v += gridDim.x * kBlock) {
-    int lo = 0, hi = R;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (keys[mid] < (uint32_t)v) lo = mid + 1;
-      else hi = mid;
-    }
-    dprefix[v] = lo;
-  }
-}
-
-// vals may be drec itself (an even number of sort passes)
-__global__ void __launch_bounds__(kBlock) k_rec_payload(const uint32_t* vals,
-                                                        const int* __restrict__ d_nrec,
-                                                        const int* __restrict__ d_nvf,
-                                                        const int4* __restrict__ vf_v,
-                                                        const float4* __restrict__ vf_w,
-                                                        const int4* __restrict__ ee_v,
-                                                        const float4* __restrict__ ee_w,
-                                                        uint32_t* drec,
-                                                        RecPayload* __restrict__ rec) {
+// From the sorted keys and values: dprefix[v] = the number of keys below v (binary
+// search, v in [0, n]), and the records and their payloads in sorted order. vals
+// may be drec itself (an even number of sort passes).
+__global__ void __launch_bounds__(kBlock) k_rec_place(int n, const uint32_t* __restrict__ keys,
+                                                      const uint32_t* vals,
+                                                      const int* __restrict__ d_nrec,
+                                                      const int* __restrict__ d_nvf,
+                                                      const int4* __restrict__ vf_v,
+                                                      const float4* __restrict__ vf_w,
+                                                      const int4* __restrict__ ee_v,
+                                                      const float4* __restrict__ ee_w,
+                                                      int* __restrict__ dprefix, uint32_t* drec,
+                                                      RecPayload* __restrict__ rec) {
   const int R = *d_nrec;
   const int nvf = *d_nvf;
-  for (int q = blockIdx.x * kBlock + threadIdx.x; q < R; q += gridDim.x * kBlock) {
-    const uint32_t r = vals[q];
-    drec[q] = r;
-    put_payload(r, q, nvf, vf_v, vf_w, ee_v, ee_w, rec);
+  const int m = max(R, n + 1);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+    if (i <= n) {
+      int lo = 0, hi = R;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint32_t)i) lo = mid + 1;
+        else hi = mid;
+      }
+      dprefix[i] = lo;
+    }
+    if (i < R) {
+      const uint32_t r = vals[i];
+      drec[i] = r;
+      put_payload(r, i, nvf, vf_v, vf_w, ee_v, ee_w, rec);
+    }
   }
 }
 
@@ -987,48 +972,45 @@ int launch_ccd_plan_only(const NarrowBufs& nb, const float4* x, const float4* v,
 }
 
 int launch_dynamic_csr(const NarrowBufs& nb, hipStream_t s) {
-  // dcount[0, n) is zero here: cleared at creation, then by every k_dyn_fill
   const int gc = grid_for(nb.vf_out_cap + nb.ee_out_cap);
-  // Two ways to the same CSR, chosen on the device from the substep's record count:
-  // per-vertex counts and atomic placement, then a per-vertex sort (at most sort_min
-  // records), or a radix sort of all records by vertex id (more). The kernels of
-  // the way not taken return at once; with sort_min = 0 the sorted way is not launched.
-  const int sm = nb.sort_min;
+  if (nb.sorted_csr) {
+    // every record keyed by its vertex, one stable radix sort, then the prefixes and
+    // payloads from the sorted keys
+    const int64_t rcap = 4 * (nb.vf_out_cap + nb.ee_out_cap);
+    uint32_t* keys = nb.skey;
+    uint32_t* kalt = nb.skey + rcap;
+    hipLaunchKernelGGL(k_rec_keys, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
+                       nb.d_nee, (uint4*)keys, (uint4*)nb.drec, nb.d_nrec);
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < nb.n) ++bits;
+    int in_alt = 0;
+    int rc = radix_sort_dev(keys, nb.drec, kalt, nb.dtmp, (int)rcap, nb.d_nrec, bits,
+                            nb.sort_temp, &in_alt, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rec_place, dim3(grid_for(std::max<int64_t>(rcap, nb.n + 1))),
+                       dim3(kBlock), 0, s, nb.n, in_alt ? kalt : keys, in_alt ? nb.dtmp : nb.drec,
+                       nb.d_nrec, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v, nb.ee_w, nb.dprefix,
+                       nb.drec, nb.rec);
+    CM_LAUNCH_CHECK();
+    return CM_OK;
+  }
+  // per-vertex counts, a scan, atomic placement, then each vertex's list ordered;
+  // dcount[0, n) is zero here: cleared at creation, then by every k_dyn_fill
   hipLaunchKernelGGL(k_dyn_count, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
-                     nb.d_nee, nb.dcount, nb.n, sm);
+                     nb.d_nee, nb.dcount, nb.n);
   CM_LAUNCH_CHECK();
   int rc = scan_exclusive(nb.dcount, nb.dprefix, nb.n, nullptr, nb.dprefix + nb.n, nb.scan_temp,
                           s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_dyn_fill, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
-                     nb.d_nee, nb.dprefix, nb.dcount, nb.drec, sm);
+                     nb.d_nee, nb.dprefix, nb.dcount, nb.drec);
   hipLaunchKernelGGL(k_dyn_sort, dim3(nblocks(nb.n)), dim3(kBlock), 0, s, nb.n, nb.dprefix,
                      nb.drec, nb.dcount + nb.n, nb.dlong, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v,
-                     nb.ee_w, nb.rec, nb.d_nee, sm);
-  // k_dyn_count zeroed the long-list count when the sorted way runs
+                     nb.ee_w, nb.rec);
   hipLaunchKernelGGL(k_dyn_sort_long, dim3(512), dim3(kBlock), 0, s, nb.dprefix, nb.dcount + nb.n,
                      nb.dlong, nb.drec, nb.dtmp, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v, nb.ee_w,
                      nb.rec);
   CM_LAUNCH_CHECK();
-  if (sm > 0) {
-    const int64_t rcap = 4 * (nb.vf_out_cap + nb.ee_out_cap);
-    uint32_t* keys = nb.skey;
-    uint32_t* kalt = nb.skey + rcap;
-    hipLaunchKernelGGL(k_rec_keys, dim3(gc), dim3(kBlock), 0, s, nb.vf_v, nb.ee_v, nb.d_nvf,
-                       nb.d_nee, sm, (uint4*)keys, (uint4*)nb.drec, nb.d_nrec);
-    int bits = 1;
-    while (bits < 32 && ((int64_t)1 << bits) < nb.n) ++bits;
-    int in_alt = 0;
-    rc = radix_sort_dev(keys, nb.drec, kalt, nb.dtmp, (int)rcap, nb.d_nrec, bits, nb.sort_temp,
-                        &in_alt, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_rec_prefix, dim3(grid_for(nb.n + 1)), dim3(kBlock), 0, s, nb.n,
-                       in_alt ? kalt : keys, nb.d_nrec, nb.dprefix);
-    hipLaunchKernelGGL(k_rec_payload, dim3(grid_for(rcap)), dim3(kBlock), 0, s,
-                       in_alt ? nb.dtmp : nb.drec, nb.d_nrec, nb.d_nvf, nb.vf_v, nb.vf_w, nb.ee_v,
-                       nb.ee_w, nb.drec, nb.rec);
-    CM_LAUNCH_CHECK();
-  }
   return CM_OK;
 }
 

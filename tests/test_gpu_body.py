@@ -483,11 +483,11 @@ def test_kernel_timing_leaves_the_step_unchanged():
 
 
 def test_sorted_record_csr_matches_atomic_csr(monkeypatch):
-    """The per-vertex record CSR built by the radix sort of all records (the way taken
-    above a record threshold, CLOTHMI_SORT_CSR) equals the one built by atomic
-    placement and per-vertex sorts: twin bodies, one on each way, step to the same
-    state bit for bit. Cases: the folded cloth, the crumpled 40 x 40 cloth (vertices
-    with long record lists) and a batch of 8 cloths."""
+    """The per-vertex record CSR built by the radix sort of all records (the default)
+    equals the one built by atomic placement and per-vertex sorts (CLOTHMI_SORT_CSR=0):
+    twin bodies, one on each way, step to the same state bit for bit. Cases: the
+    folded cloth, the crumpled 40 x 40 cloth (vertices with long record lists) and a
+    batch of 8 cloths."""
     from licenta_amd.body import ClothBody
     n = 24
     fold = configs.config_scene(0, n)
