@@ -802,7 +802,7 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
   // the same with any budget.
   if ((rc = dalloc(b, &b->lb.spill, (size_t)t * kTravSpill))) return fail(rc);
   {
-    int budget = params->pair_cap <= 96 ? 160 : 0;
+    int budget = params->pair_cap <= 96 ? 240 : 0;
     if (const char* e = std::getenv("CLOTHMI_TRAV_BUDGET")) budget = std::max(0, std::atoi(e));
     if (budget > 0 && t >= 2 && t < (1 << 26)) {
       b->lb.rcap = t / 4 + 1024;

@@ -61,9 +61,11 @@ int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_to
 // Exclusive scan of per-tile sums (tiles of per_tile items; ceil(*d_items / per_tile)
 // of them, at most ntile_max; all ntile_max when d_items is null). packed: each sum
 // holds two counts below 2^16, scanned into pre[2i] (low) and pre[2i+1] (high);
-// else pre[i]. The totals, clamped to cap0 / cap1, go to *d_tot0 / *d_tot1.
+// else pre[i]. The totals (below 2^31), clamped to cap0 / cap1, go to *d_tot0 /
+// *d_tot1. temp: a scan temp (scan_temp_bytes(n) with n >= ntile_max).
 int scan_tiles(const int* d_items, int per_tile, int ntile_max, const int* sums, int* pre,
-               int packed, int64_t cap0, int64_t cap1, int* d_tot0, int* d_tot1, hipStream_t s);
+               int packed, int64_t cap0, int64_t cap1, int* d_tot0, int* d_tot1, void* temp,
+               hipStream_t s);
 size_t radix_sort_temp_bytes(int n);
 int radix_sort(uint64_t* keys, uint32_t* vals, int n, int key_bits, void* temp, hipStream_t s);
 int radix_sort_clear(void* temp, int n, hipStream_t s);  // before the first sort on a temp
@@ -97,7 +99,7 @@ struct LbvhBufs {
   float4 *tmin = nullptr, *tmax = nullptr;  // deferred-node scratch (large trees)
   int *trm = nullptr, *tdone = nullptr;
   float4* nodes = nullptr;  // 4 float4 per internal node (children, rightmost leaves, boxes)
-  int trav_chunk = 2;       // traversal: tiles per chunk dealt to one XCD (launch shape only)
+  int trav_chunk = 4;       // traversal: tiles per chunk dealt to one XCD (launch shape only)
   // traversal: visits a leaf makes in the first pass before a wave takes it over (0:
   // one pass); records of kTravRec ints for up to rcap such leaves; hctl[0] their
   // count, zero between substeps (the pair compaction clears it). The pair lists are

@@ -1260,7 +1260,7 @@ static int launch_traverse(const LbvhBufs& b, int cap, int2* chunk, int* counts,
                            hipStream_t s) {
   const int t = b.t;
   const int nt = (t + kTravBlock - 1) / kTravBlock;
-  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 2;
+  const int xchunk = b.trav_chunk > 0 ? b.trav_chunk : 4;
   const int round = kXcds * xchunk;
   const bool two = b.trav_budget > 0 && b.trec && b.hctl && b.rcap > 0;
   hipLaunchKernelGGL(k_traverse, dim3((nt + round - 1) / round * round), dim3(kTravBlock), 0, s, t,
@@ -1292,7 +1292,7 @@ int launch_pairs(const LbvhBufs& b, int cap, int2* chunk, int* counts, int* offs
   int rc = launch_traverse(b, cap, chunk, counts, offsets, s);
   if (rc) return rc;
   rc = scan_tiles(nullptr, 1, nt, offsets, offsets + nt, 0, (int64_t)t * cap, 0, d_npairs,
-                  nullptr, s);
+                  nullptr, scan_temp, s);
   if (rc) return rc;
   const bool two = b.trav_budget > 0 && b.trec && b.hctl && b.rcap > 0;
   hipLaunchKernelGGL(k_compact_pairs, dim3(nt), dim3(kBlock), 0, s, t, cap, chunk, counts,

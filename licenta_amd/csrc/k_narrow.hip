@@ -926,7 +926,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
                      nb.d_npairs, nb.pair_cap, tri, x, p, th, ee_on, nb.mask, nb.tile_sum);
   CM_LAUNCH_CHECK();
   int rc = scan_tiles(nb.d_npairs, 64 * kSmall, ptiles, nb.tile_sum, nb.tile_pre, 1, nb.vf_cap,
-                      nb.ee_cap, nb.d_ncand_vf, nb.d_ncand_ee, s);
+                      nb.ee_cap, nb.d_ncand_vf, nb.d_ncand_ee, nb.scan_temp, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_cand_write, dim3(tile_grid(nb.pair_cap, kSmall)), dim3(kBlock), 0, s, nb.pairs,
                      nb.d_npairs, nb.pair_cap, tri, nb.mask, nb.tile_pre, nb.cand_vf, nb.cand_ee,
@@ -940,7 +940,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   CM_LAUNCH_CHECK();
   const int ctiles = (int)((ccap + 64 * kBig - 1) / (64 * kBig));
   rc = scan_tiles(nb.d_ntotal, 64 * kBig, ctiles, nb.tile_sum, nb.tile_pre, 0, ccap, 0,
-                  nb.d_nnear, nullptr, s);
+                  nb.d_nnear, nullptr, nb.scan_temp, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_near_list, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.d_ntotal,
                      nb.hit_flag, nb.tile_pre, nb.near_ids);
@@ -950,7 +950,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   CM_LAUNCH_CHECK();
   const int ntiles_near = (int)((ccap + 64 * kSmall - 1) / (64 * kSmall));
   rc = scan_tiles(nb.d_nnear, 64 * kSmall, ntiles_near, nb.tile_sum, nb.tile_pre, 1, nb.vf_out_cap,
-                  nb.ee_out_cap, nb.d_nvf, nb.d_nee, s);
+                  nb.ee_out_cap, nb.d_nvf, nb.d_nee, nb.scan_temp, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_ccd_write, dim3(tile_grid(ccap, kSmall)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,

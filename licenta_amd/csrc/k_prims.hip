@@ -362,105 +362,184 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter_dev(
 
 // ---- scan of per-tile sums ----------------------------------------------------------
 // Kernels that produce counts in tiles of consecutive items also write each tile's sum;
-// this one workgroup scans the sums into tile prefixes, and the consuming kernel
-// adds the exclusive scan within its tile. The counts are then read once instead
-// of three times. A packed sum holds two counts below 2^16 (low and high half),
-// scanned separately; the totals are clamped to the buffers they index.
-constexpr int kTileScanBlock = 1024;
-constexpr int kTileScanChunk = 8192;  // sums staged in LDS per round (two 32 KiB halves)
+// this scan turns the sums into tile prefixes, and the consuming kernel adds the
+// exclusive scan within its tile. The counts are then read once instead of three
+// times. A packed sum holds two counts below 2^16 (low and high half), scanned
+// separately; the totals are clamped to the buffers they index. It is the
+// single-pass look-back scan above carried over to a pair of counts: a thread adds
+// both as one 64-bit sum (first count in the low word), and a tile state holds the
+// two running counts in 31 bits each (every count and total is below 2^31).
+__device__ __forceinline__ unsigned long long wave_incl_scan64(unsigned long long v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_reduce64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned long long pair_of(int v, int packed) {
+  return packed ? (unsigned long long)(v & 0xffff) | ((unsigned long long)((unsigned)v >> 16) << 32)
+                : (unsigned long long)(unsigned)v;
+}
+// 64-bit pair sum <-> the 62 value bits of a tile state
+__device__ __forceinline__ unsigned long long pack31(unsigned long long x) {
+  return (x & 0x7fffffffull) | (((x >> 32) & 0x7fffffffull) << 31);
+}
+__device__ __forceinline__ unsigned long long unpack31(unsigned long long st) {
+  return (st & 0x7fffffffull) | (((st >> 31) & 0x7fffffffull) << 32);
+}
 
-__global__ void __launch_bounds__(kTileScanBlock) k_scan_tiles(
-    const int* __restrict__ d_items, int per_tile, int ntile_max, const int* __restrict__ sums,
-    int* __restrict__ pre, int packed, int64_t cap0, int64_t cap1, int* __restrict__ d_tot0,
-    int* __restrict__ d_tot1) {
-  // Each round stages up to kTileScanChunk sums in LDS with coalesced loads, scans
-  // them (8 consecutive sums per thread, then across the workgroup) and writes the
-  // prefixes back with coalesced stores.
-  __shared__ int sa[kTileScanChunk], sb[kTileScanChunk];
-  __shared__ int lds[2][kTileScanBlock / 64];
-  constexpr int kPer = kTileScanChunk / kTileScanBlock;  // 8
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(
+    const int* __restrict__ d_items, int per_tile, int ntile_max, const int* sums, int* pre,
+    int packed, int64_t cap0, int64_t cap1, int* __restrict__ d_tot0, int* __restrict__ d_tot1,
+    int* __restrict__ ctr, unsigned long long* __restrict__ state) {
+  __shared__ unsigned long long lds[kBlock / 64 + 1];
+  __shared__ unsigned long long s_prefix;
+  __shared__ int s_tile, s_last;
   const int items = d_items ? max(0, *d_items) : ntile_max * per_tile;
-  const int nt = min(ntile_max, (items + per_tile - 1) / per_tile);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int carry_a = 0, carry_b = 0;
-  for (int c0 = 0; c0 < nt; c0 += kTileScanChunk) {
-    const int len = min(kTileScanChunk, nt - c0);
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int i = k * kTileScanBlock + (int)threadIdx.x;
-      const int v = i < len ? sums[c0 + i] : 0;
-      sa[i] = packed ? (v & 0xffff) : v;
-      sb[i] = packed ? (v >> 16) : 0;
-    }
+  const int n = min(ntile_max, (items + per_tile - 1) / per_tile);
+  const int ntiles = (n + kScanTile - 1) / kScanTile;
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (d_tot0) *d_tot0 = 0;
+    if (packed && d_tot1) *d_tot1 = 0;
+  }
+  const bool vin = (reinterpret_cast<uintptr_t>(sums) & 15) == 0;
+  const bool vout = (reinterpret_cast<uintptr_t>(pre) & 15) == 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(&ctr[0], 1);
     __syncthreads();
-    const int base = (int)threadIdx.x * kPer;
-    int a = 0, b = 0;
+    const int tile = s_tile;
+    if (tile >= ntiles) break;
+    const int base = tile * kScanTile + threadIdx.x * kScanItems;
+    const bool full = base + kScanItems <= n;
+    int v[kScanItems];
+    if (vin && full) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      a += sa[base + k];
-      b += sb[base + k];
-    }
-    int ia = a, ib = b;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int ta = __shfl_up(ia, o, 64), tb = __shfl_up(ib, o, 64);
-      if (lane >= o) {
-        ia += ta;
-        ib += tb;
+      for (int q = 0; q < kScanItems / 4; ++q) {
+        const int4 t4 = reinterpret_cast<const int4*>(sums + base)[q];
+        v[4 * q] = t4.x; v[4 * q + 1] = t4.y; v[4 * q + 2] = t4.z; v[4 * q + 3] = t4.w;
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? sums[base + k] : 0;
     }
-    if (lane == 63) {
-      lds[0][wid] = ia;
-      lds[1][wid] = ib;
-    }
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += pair_of(v[k], packed);
+    // exclusive scan of the thread sums over the workgroup
+    const unsigned long long inc = wave_incl_scan64(sum);
+    if (lane == 63) lds[wid] = inc;
     __syncthreads();
-    int ra = carry_a + ia - a, rb = carry_b + ib - b;
-    for (int w = 0; w < kTileScanBlock / 64; ++w) {
-      const int ta = lds[0][w], tb = lds[1][w];
-      if (w < wid) {
-        ra += ta;
-        rb += tb;
+    if (threadIdx.x == 0) {
+      unsigned long long run = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        const unsigned long long t = lds[w];
+        lds[w] = run;
+        run += t;
       }
-      carry_a += ta;
-      carry_b += tb;
-    }
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int va = sa[base + k], vb = sb[base + k];
-      sa[base + k] = ra;
-      sb[base + k] = rb;
-      ra += va;
-      rb += vb;
+      lds[kBlock / 64] = run;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int i = k * kTileScanBlock + (int)threadIdx.x;
-      if (i < len) {
-        if (packed) {
-          pre[2 * (c0 + i)] = sa[i];
-          pre[2 * (c0 + i) + 1] = sb[i];
-        } else {
-          pre[c0 + i] = sa[i];
+    const unsigned long long ex = inc - sum + lds[wid];
+    const unsigned long long total = lds[kBlock / 64];
+    if (threadIdx.x < 64) {
+      unsigned long long prefix = 0;
+      if (tile > 0) {
+        if (lane == 0) st_state(&state[tile], kFlagAgg | pack31(total));
+        int j = tile - 1;
+        for (;;) {
+          const int idx = j - lane;  // lane 0 is the nearest predecessor
+          const unsigned long long st = idx >= 0 ? ld_state(&state[idx]) : kFlagInc;
+          const unsigned flag = (unsigned)(st >> 62);
+          const unsigned long long pm = __ballot(flag == 2u), zm = __ballot(flag == 0u);
+          const int first = pm ? __ffsll((long long)pm) - 1 : 64;
+          const unsigned long long need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
+          if (zm & need) continue;  // a predecessor up to the first prefix is still empty
+          prefix += wave_reduce64(lane <= first ? unpack31(st) : 0ull);
+          if (first < 64) break;
+          j -= 64;
         }
       }
+      if (lane == 0) {
+        st_state(&state[tile], kFlagInc | pack31(prefix + total));
+        s_prefix = prefix;
+      }
+    }
+    __syncthreads();
+    unsigned long long run = s_prefix + ex;
+    if (vout && full) {
+      if (packed) {
+#pragma unroll
+        for (int q = 0; q < kScanItems / 2; ++q) {
+          int4 o;
+          o.x = (int)(unsigned)run;
+          o.y = (int)(run >> 32);
+          run += pair_of(v[2 * q], 1);
+          o.z = (int)(unsigned)run;
+          o.w = (int)(run >> 32);
+          run += pair_of(v[2 * q + 1], 1);
+          reinterpret_cast<int4*>(pre + 2 * base)[q] = o;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+          int4 o;
+          o.x = (int)run; run += (unsigned)v[4 * q];
+          o.y = (int)run; run += (unsigned)v[4 * q + 1];
+          o.z = (int)run; run += (unsigned)v[4 * q + 2];
+          o.w = (int)run; run += (unsigned)v[4 * q + 3];
+          reinterpret_cast<int4*>(pre + base)[q] = o;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) {
+          if (packed) {
+            pre[2 * (base + k)] = (int)(unsigned)run;
+            pre[2 * (base + k) + 1] = (int)(run >> 32);
+          } else {
+            pre[base + k] = (int)(unsigned)run;
+          }
+          run += pair_of(v[k], packed);
+        }
+    }
+    if (tile == ntiles - 1 && threadIdx.x == 0) {
+      const unsigned long long all = s_prefix + total;
+      if (d_tot0) *d_tot0 = (int)min<int64_t>((int64_t)(all & 0xffffffffull), cap0);
+      if (packed && d_tot1) *d_tot1 = (int)min<int64_t>((int64_t)(all >> 32), cap1);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    if (d_tot0) *d_tot0 = (int)min<int64_t>(carry_a, cap0);
-    if (packed && d_tot1) *d_tot1 = (int)min<int64_t>(carry_b, cap1);
+    __threadfence();
+    s_last = atomicAdd(&ctr[1], 1) == (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last) {
+    for (int i = threadIdx.x; i < ntiles; i += kBlock) st_state(&state[i], 0ull);
+    if (threadIdx.x == 0) {
+      ctr[0] = 0;
+      ctr[1] = 0;
+    }
   }
 }
 
 }  // namespace
 
-static int scan_tiles(int64_t n_max) {
+static int lookback_tiles(int64_t n_max) {
   return (int)std::max<int64_t>(1, (n_max + kScanTile - 1) / kScanTile);
 }
 
 size_t scan_temp_bytes(int n_max) {
-  return 256 + align_up(sizeof(unsigned long long) * (size_t)scan_tiles(n_max));
+  return 256 + align_up(sizeof(unsigned long long) * (size_t)lookback_tiles(n_max));
 }
 
 int scan_temp_clear(void* temp, int n_max, hipStream_t s) {
@@ -471,7 +550,7 @@ int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_to
                    hipStream_t s) {
   int* ctr = (int*)temp;
   unsigned long long* state = (unsigned long long*)((char*)temp + 256);
-  const int grid = std::min(scan_tiles(n_max), kScanMaxGrid);
+  const int grid = std::min(lookback_tiles(n_max), kScanMaxGrid);
   hipLaunchKernelGGL(k_scan, dim3(grid), dim3(kBlock), 0, s, in, out, n_max, d_n, d_total, ctr,
                      state);
   CM_LAUNCH_CHECK();
@@ -479,9 +558,13 @@ int scan_exclusive(const int* in, int* out, int n_max, const int* d_n, int* d_to
 }
 
 int scan_tiles(const int* d_items, int per_tile, int ntile_max, const int* sums, int* pre,
-               int packed, int64_t cap0, int64_t cap1, int* d_tot0, int* d_tot1, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kTileScanBlock), 0, s, d_items, per_tile,
-                     ntile_max, sums, pre, packed, cap0, cap1, d_tot0, d_tot1);
+               int packed, int64_t cap0, int64_t cap1, int* d_tot0, int* d_tot1, void* temp,
+               hipStream_t s) {
+  int* ctr = (int*)temp;
+  unsigned long long* state = (unsigned long long*)((char*)temp + 256);
+  const int grid = std::min(lookback_tiles(ntile_max), kScanMaxGrid);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(grid), dim3(kBlock), 0, s, d_items, per_tile, ntile_max,
+                     sums, pre, packed, cap0, cap1, d_tot0, d_tot1, ctr, state);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
