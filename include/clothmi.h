@@ -310,7 +310,7 @@ int cm_adjoint(cm_body* body, const float* dl_dx, const float* dl_dv, float* dl_
  * No counterpart in the reference (its launch shapes are fixed). Times `substeps`
  * substeps from the current state for each candidate of, one at a time: the Jacobi
  * tile size (64, 128 or 256 vertices), the traversal's tiles per XCD chunk (1, 2, 4,
- * 8) and its first pass's visit budget (128, 160, 240; kept as it is when the
+ * 8) and its first pass's visit budget (128, 160, 200, 240; kept as it is when the
  * two-pass traversal is off), keeps the fastest of each,
  * and restores the state (the simulation does not advance). None of them changes a
  * result. chosen (3 ints, may be NULL) receives the picks in that order. Syncs; the

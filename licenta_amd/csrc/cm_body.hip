@@ -802,7 +802,7 @@ int create_impl(const cm_params* params, const float* xyz, int32_t n, const uint
   // the same with any budget.
   if ((rc = dalloc(b, &b->lb.spill, (size_t)t * kTravSpill))) return fail(rc);
   {
-    int budget = params->pair_cap <= 96 ? 240 : 0;
+    int budget = params->pair_cap <= 96 ? 200 : 0;
     if (const char* e = std::getenv("CLOTHMI_TRAV_BUDGET")) budget = std::max(0, std::atoi(e));
     if (budget > 0 && t >= 2 && t < (1 << 26)) {
       b->lb.rcap = t / 4 + 1024;
@@ -1731,7 +1731,7 @@ int cm_autotune(cm_body* b, int32_t substeps, int32_t* chosen) {
     b->lb.trav_chunk = best;
   }
   if (b->lb.trav_budget > 0) {  // the two-pass traversal is on
-    const int cand[3] = {128, 160, 240};
+    const int cand[4] = {128, 160, 200, 240};
     int best = b->lb.trav_budget;
     float best_ms = 1e30f;
     for (int c : cand) {

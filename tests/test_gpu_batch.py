@@ -125,7 +125,7 @@ def test_autotune_keeps_results():
     assert np.array_equal(b.normals(), n_before)
     assert picks["tile_vertices"] in (64, 128, 256)
     assert picks["traversal_chunk"] in (1, 2, 4, 8)
-    assert picks["traversal_budget"] in (128, 160, 240)
+    assert picks["traversal_budget"] in (128, 160, 200, 240)
     a.fixed_update(4)
     b.fixed_update(4)
     assert np.array_equal(a.positions(), b.positions())
