@@ -630,7 +630,8 @@ __global__ void __launch_bounds__(kBlock) k_ccd_test(
     const int* __restrict__ near_ids, const int* __restrict__ plan,
     const float2* __restrict__ cps, const float* __restrict__ roots,
     const float4* __restrict__ x, const float4* __restrict__ v, float th, float dt, int dcd,
-    uint8_t* __restrict__ hit_flag, int* __restrict__ tile_sum) {
+    uint8_t* __restrict__ hit_flag, int* __restrict__ tile_sum, float4* __restrict__ hit_data,
+    int64_t hit_cap) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nn = *d_nnear;
   const int ntiles = (nn + (64 * kSmall) - 1) / (64 * kSmall);
   const int lane = threadIdx.x & 63;
@@ -648,6 +649,10 @@ __global__ void __launch_bounds__(kBlock) k_ccd_test(
         const bool hit = ccd_tests(isvf, q, plan[id], cps[id], id, roots, x, v, th, dt, dcd, h);
         hit_flag[k] = (uint8_t)hit;
         if (hit) c += isvf ? 1 : (1 << 16);
+        if (hit && k < hit_cap) {  // kept for k_ccd_write
+          hit_data[2 * (int64_t)k] = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+          hit_data[2 * (int64_t)k + 1] = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+        }
       }
     }
     c = wave_sum(c);
@@ -655,8 +660,9 @@ __global__ void __launch_bounds__(kBlock) k_ccd_test(
   }
 }
 
-// pass 6 (after the scan of the tile counts): each hit is recomputed and written at
-// its ordered position among the VF or EE contacts.
+// pass 6 (after the scan of the tile counts): each hit is written at its ordered
+// position among the VF or EE contacts, with the weights, time and normal k_ccd_test
+// kept for it (recomputed for near indices at or past hit_cap).
 __global__ void __launch_bounds__(kBlock) k_ccd_write(
     const int4* __restrict__ cand_vf, const int4* __restrict__ cand_ee,
     const int* __restrict__ d_nvf, int64_t vf_cap, const int* __restrict__ d_nnear,
@@ -666,7 +672,8 @@ __global__ void __launch_bounds__(kBlock) k_ccd_write(
     int64_t ee_out_cap, const float4* __restrict__ x, const float4* __restrict__ v, float th,
     float dt, int dcd, int4* __restrict__ vf_v, float4* __restrict__ vf_w,
     float4* __restrict__ vf_tn, int4* __restrict__ ee_v, float4* __restrict__ ee_w,
-    float4* __restrict__ ee_tn, uint32_t* __restrict__ warn) {
+    float4* __restrict__ ee_tn, uint32_t* __restrict__ warn,
+    const float4* __restrict__ hit_data, int64_t hit_cap) {
   const int nvf = (int)min((int64_t)*d_nvf, vf_cap), nn = *d_nnear;
   const int ntiles = (nn + (64 * kSmall) - 1) / (64 * kSmall);
   const int lane = threadIdx.x & 63;
@@ -693,10 +700,17 @@ __global__ void __launch_bounds__(kBlock) k_ccd_write(
         over = true;
         continue;
       }
-      Hit h;
-      ccd_tests(isvf, q, plan[id[j]], cps[id[j]], id[j], roots, x, v, th, dt, dcd, h);
-      const float4 w4 = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
-      const float4 tn = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+      const int64_t k = (int64_t)tile * (64 * kSmall) + j * 64 + lane;
+      float4 w4, tn;
+      if (k < hit_cap) {
+        w4 = hit_data[2 * k];
+        tn = hit_data[2 * k + 1];
+      } else {
+        Hit h;
+        ccd_tests(isvf, q, plan[id[j]], cps[id[j]], id[j], roots, x, v, th, dt, dcd, h);
+        w4 = make_float4(h.w[0], h.w[1], h.w[2], h.w[3]);
+        tn = make_float4(h.t, h.n.x, h.n.y, h.n.z);
+      }
       if (isvf) { vf_v[o] = q; vf_w[o] = w4; vf_tn[o] = tn; }
       else { ee_v[o] = q; ee_w[o] = w4; ee_tn[o] = tn; }
     }
@@ -944,9 +958,13 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
   if (rc) return rc;
   hipLaunchKernelGGL(k_near_list, dim3(tile_grid(ccap, kBig)), dim3(kBlock), 0, s, nb.d_ntotal,
                      nb.hit_flag, nb.tile_pre, nb.near_ids);
+  // the record payloads (32 B each) are free until k_rec_place refills them: they keep
+  // each hit's weights, time and normal from k_ccd_test for k_ccd_write
+  const int64_t hit_cap = 4 * (nb.vf_out_cap + nb.ee_out_cap);
   hipLaunchKernelGGL(k_ccd_test, dim3(tile_grid(ccap, kSmall)), dim3(kBlock), 0, s, nb.cand_vf,
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,
-                     nb.ccp, nb.croot, x, v, th, dt, dcd, nb.hit_flag, nb.tile_sum);
+                     nb.ccp, nb.croot, x, v, th, dt, dcd, nb.hit_flag, nb.tile_sum,
+                     (float4*)nb.rec, hit_cap);
   CM_LAUNCH_CHECK();
   const int ntiles_near = (int)((ccap + 64 * kSmall - 1) / (64 * kSmall));
   rc = scan_tiles(nb.d_nnear, 64 * kSmall, ntiles_near, nb.tile_sum, nb.tile_pre, 1, nb.vf_out_cap,
@@ -956,7 +974,7 @@ int launch_narrow(const NarrowBufs& nb, const int4* tri, const float4* x, const 
                      nb.cand_ee, nb.d_ncand_vf, nb.vf_cap, nb.d_nnear, nb.near_ids, nb.cplan,
                      nb.ccp, nb.croot, nb.hit_flag, nb.tile_pre, nb.vf_out_cap, nb.ee_out_cap, x,
                      v, th, dt, dcd, nb.vf_v, nb.vf_w, nb.vf_tn, nb.ee_v, nb.ee_w, nb.ee_tn,
-                     nb.warn);
+                     nb.warn, (const float4*)nb.rec, hit_cap);
   CM_LAUNCH_CHECK();
   return CM_OK;
 }
