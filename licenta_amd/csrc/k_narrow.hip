@@ -603,22 +603,41 @@ __global__ void __launch_bounds__(kBlock) k_near_list(const int* __restrict__ d_
                                                       const uint8_t* __restrict__ near_flag,
                                                       const int* __restrict__ tile_pre,
                                                       int* __restrict__ near_ids) {
+  // lane l takes the kBig consecutive ids from tile * 64 * kBig + l * kBig (one 8-byte
+  // load of their flags), so the ids leave in order: lane by lane, then within a lane
+  static_assert(kBig == 8, "one 8-byte load of near flags per lane");
   const int total = *d_ntotal;
   const int ntiles = (total + (64 * kBig) - 1) / (64 * kBig);
   const int lane = threadIdx.x & 63;
   for (int tile = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6); tile < ntiles;
        tile += gridDim.x * kWavesPerBlock) {
-    int c[kBig], off[kBig];
+    const int b0 = tile * (64 * kBig) + lane * kBig;
+    uint32_t lo = 0, hi = 0;
+    if (b0 + kBig <= total) {
+      const uint2 u = *reinterpret_cast<const uint2*>(near_flag + b0);
+      lo = u.x;
+      hi = u.y;
+    } else {
+#pragma unroll
+      for (int j = 0; j < kBig; ++j) {
+        const uint32_t f = b0 + j < total ? near_flag[b0 + j] : 0u;
+        if (j < 4) lo |= f << (8 * j);
+        else hi |= f << (8 * (j - 4));
+      }
+    }
+    const int cnt = __popc(lo & 0x01010101u) + __popc(hi & 0x01010101u);
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    int at = tile_pre[tile] + inc - cnt;
 #pragma unroll
     for (int j = 0; j < kBig; ++j) {
-      const int id = tile * (64 * kBig) + j * 64 + lane;
-      c[j] = id < total ? near_flag[id] : 0;
+      const uint32_t f = ((j < 4 ? lo : hi) >> (8 * (j & 3))) & 1u;
+      if (f) near_ids[at++] = b0 + j;
     }
-    tile_offsets<kBig>(c, off);
-    const int pre = tile_pre[tile];
-#pragma unroll
-    for (int j = 0; j < kBig; ++j)
-      if (c[j]) near_ids[pre + off[j]] = tile * (64 * kBig) + j * 64 + lane;
   }
 }
 
