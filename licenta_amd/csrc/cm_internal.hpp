@@ -181,7 +181,9 @@ struct NarrowBufs {
   int *dcount = nullptr, *dprefix = nullptr, *dcursor = nullptr;  // n + 1, n + 1, n
   uint32_t* drec = nullptr;  // 4 * (vf_out_cap + ee_out_cap)
   uint32_t* dtmp = nullptr;  // same size: staging for very long record lists
-  RecPayload* rec = nullptr;  // same size: record payloads in drec order
+  // same size: record payloads in drec order; between the solver's last read and
+  // k_rec_place it holds the CCD hits' weights, time and normal per near index
+  RecPayload* rec = nullptr;
   int* dlong = nullptr;      // n: vertices whose record lists are ordered by a workgroup
   // sorted_csr: the CSR comes from one radix sort of the records by vertex id
   // (else from per-vertex counts and atomic placement). skey: record keys then
